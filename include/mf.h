@@ -1,0 +1,243 @@
+/*
+ * mf.h - C ABI of the MI355X (gfx950) push-relabel maximum-flow library (libmf.so).
+ *
+ * The reference solver has no library or FFI layer: its host driver calls C++
+ * functions on raw int arrays.  Each entry point below replaces one of those
+ * functions (or the CLI step that calls it); the reference file:line it
+ * replaces is given beside it.  All paths are relative to the reference tree.
+ *
+ * Conventions
+ *   - Plain C, no exceptions, no exit(): every call returns an mf_status.
+ *   - Host memory passed in is owned by the caller and only read during the
+ *     call unless documented otherwise.  Device memory is owned by the library.
+ *   - All capacities, excesses and heights are int32; flow values are reported
+ *     as int64.  Sizes that do not fit the int32 device representation return
+ *     MF_ERR_OVERFLOW instead of wrapping.
+ *   - One mf_context per host thread; contexts are independent.
+ */
+#ifndef MF_H
+#define MF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MF_VERSION_MAJOR 0
+#define MF_VERSION_MINOR 1
+
+/* Replaces the CHECK(x) -> exit(1) error path of maxflow-cuda/include/utils.cuh:18-26
+ * and the print-and-continue paths of the loaders (maxflow-cuda/src/graph.cpp:153-156). */
+typedef enum mf_status {
+  MF_OK = 0,
+  MF_ERR_INVALID_ARGUMENT = 1, /* bad pointer, s == t, s/t out of range, bad enum */
+  MF_ERR_IO = 2,               /* file missing or unreadable */
+  MF_ERR_PARSE = 3,            /* malformed file contents */
+  MF_ERR_OVERFLOW = 4,         /* a size or an excess does not fit int32 */
+  MF_ERR_DEVICE = 5,           /* a HIP runtime call failed */
+  MF_ERR_NO_DEVICE = 6,        /* no usable GPU */
+  MF_ERR_OUT_OF_MEMORY = 7,    /* device or host allocation failed */
+  MF_ERR_ITERATION_LIMIT = 8,  /* solver exceeded mf_options.max_rounds */
+  MF_ERR_INTERNAL = 9,         /* a solver invariant failed (e.g. residual s->t path at exit) */
+  MF_ERR_BUDGET = 10,          /* serving: request larger than the whole HBM budget */
+  MF_ERR_NOT_READY = 11,       /* serving: polled request has not finished */
+  MF_ERR_NO_PATH = 12          /* reserved for callers that treat "t unreachable" as an error */
+} mf_status;
+
+/* -a flag of maxflow-cuda/src/main.cu:68-70,96-103. */
+typedef enum mf_algo {
+  MF_ALGO_TC = 0, /* thread-centric: one lane per active vertex (push_relabel_kernel.cu:14-158) */
+  MF_ALGO_VC = 1  /* vertex-centric: a lane tile per active vertex (push_relabel_kernel.cu:400-537) */
+} mf_algo;
+
+/* Residual layouts; the reference picks one by directory (maxflow-cuda, maxflow-bcsr, maxflow-bcsr2). */
+typedef enum mf_layout {
+  MF_LAYOUT_RCSR = 0,  /* forward CSR + reverse CSR with flow index (maxflow-cuda/src/graph.cpp:317-402) */
+  MF_LAYOUT_BCSR = 1,  /* one CSR with forward and reverse slots (maxflow-bcsr/src/graph.cpp:322-365);
+                          stored as interleaved {dst, residual} pairs + a reverse-slot array */
+  MF_LAYOUT_BCSR2 = 2  /* BCSR with interleaved {residual, reverse slot} pairs (maxflow-bcsr2/src/graph.cpp:322-365) */
+} mf_layout;
+
+/* Read-only view of a directed graph in CSR form (CSRGraph, maxflow-cuda/include/graph.h:28-58). */
+typedef struct mf_csr {
+  int32_t n;                /* vertices */
+  int64_t m;                /* arcs */
+  const int32_t *offsets;   /* n+1 */
+  const int32_t *dst;       /* m, 0-based */
+  const int32_t *cap;       /* m, >= 0; NULL means unit capacities (SNAP inputs) */
+  int32_t s, t;             /* source and sink, 0-based */
+} mf_csr;
+
+/* Host graph owned by the library, produced by the loaders and generators. */
+typedef struct mf_host_graph {
+  int32_t n;
+  int64_t m;
+  int32_t *offsets;         /* n+1 */
+  int32_t *dst;             /* m */
+  int32_t *cap;             /* m */
+  int32_t s, t;             /* -1 when the file does not name them (SNAP) */
+  int64_t m_header;         /* arc count declared by the file header */
+  int64_t m_processed;      /* arc lines read (CSRGraph::num_edges_processed) */
+} mf_host_graph;
+
+typedef struct mf_options {
+  int32_t cycles_per_relabel; /* push-relabel cycles between global relabels; 0 = automatic */
+  int32_t tile_lanes;         /* VC lanes per vertex: 0 = by degree bucket, else 1/8/16/32/64 */
+  int64_t max_rounds;         /* global-relabel rounds before MF_ERR_ITERATION_LIMIT; 0 = default */
+  int32_t verbose;            /* 1: print one line per round like push_relabel.cu:121,194 */
+  int32_t profile_kernels;    /* 1: time every push-relabel launch with HIP events (adds overhead) */
+  int32_t bfs_block_frontier; /* frontier size up to which the one-workgroup BFS runs; 0 = default */
+  int32_t reserved[8];
+} mf_options;
+
+typedef struct mf_result {
+  int64_t value;              /* max-flow value = excess at the sink */
+  int64_t source_outflow;     /* excess pushed out of the source by the preflow */
+  double solve_ms;            /* device solve, inputs resident in HBM */
+  double kernel_ms;           /* push-relabel cycle kernels (HIP events around each batch of cycles) */
+  double relabel_ms;          /* global relabel (BFS) */
+  double upload_ms;           /* host layout build + host->device copy */
+  int64_t rounds;             /* global relabels */
+  int64_t cycles;             /* push-relabel cycle launches */
+  int64_t active_total;       /* sum over cycles of active vertices processed */
+  int64_t slots_scanned;      /* sum over cycles of residual slots scanned */
+  int64_t pushes;
+  int64_t relabels;
+  int64_t bfs_slots;          /* slots scanned by the global relabels */
+  int64_t bytes_algorithmic;  /* algorithmic bytes of the push-relabel kernels */
+  int64_t bytes_bfs;          /* algorithmic bytes of the global relabels */
+  double profiled_kernel_ms;  /* with profile_kernels: summed per-launch event time */
+  int64_t profiled_launches;  /* with profile_kernels: number of timed launches */
+  int32_t status;
+  int32_t reserved[7];
+} mf_result;
+
+typedef struct mf_context mf_context;
+typedef struct mf_graph mf_graph; /* device-resident residual graph */
+
+/* ---- library ---- */
+const char *mf_status_string(mf_status s);
+const char *mf_version(void);
+void mf_options_default(mf_options *opts);
+
+/* ---- context: owns one HIP stream and the autotune cache ----
+ * Replaces the device setup in maxflow-cuda/src/push_relabel.cu:66-77. */
+mf_status mf_context_create(int device, mf_context **out);
+mf_status mf_context_destroy(mf_context *ctx);
+mf_status mf_device_synchronize(mf_context *ctx);
+/* Number of compute units and device name; for reports. */
+mf_status mf_device_info(mf_context *ctx, int32_t *num_cu, char *name, size_t name_len);
+
+/* ---- loaders (CSRGraph, maxflow-cuda/src/graph.cpp) ---- */
+/* CSRGraph::buildFromTxtFile, graph.cpp:28-67: '# Nodes: N Edges: M' header, unit capacities,
+ * per-vertex sorted adjacency. */
+mf_status mf_load_snap_txt(const char *path, mf_host_graph **out);
+/* CSRGraph::loadFromBinary, graph.cpp:292-314: int32 [n][m][m_processed][dst*m_processed][off*(n+1)]. */
+mf_status mf_load_snap_bin(const char *path, mf_host_graph **out);
+/* CSRGraph::saveToBinary, graph.cpp:219-233 (same layout; capacities are not stored). */
+mf_status mf_save_snap_bin(const char *path, const mf_csr *g);
+/* CSRGraph::buildFromDIMACSFile, graph.cpp:151-214: 1-based, s/t from 'n' lines, file order per vertex. */
+mf_status mf_load_dimacs(const char *path, mf_host_graph **out);
+/* DIMACS writer ('p max', 'n', 'a' lines), 1-based. */
+mf_status mf_save_dimacs(const char *path, const mf_csr *g);
+void mf_host_graph_free(mf_host_graph *g);
+mf_csr mf_host_graph_view(const mf_host_graph *g);
+
+/* ---- seeded generators (network/DIMACS/washington, genrmf, GTgraph/R-MAT) ---- */
+/* Washington RLevel(rows, levels, r): washington.c:243-279. */
+mf_status mf_gen_washington_rlg(int32_t rows, int32_t levels, int32_t r, uint64_t seed, mf_host_graph **out);
+/* Genrmf frames: genrmf.c:76-161. */
+mf_status mf_gen_genrmf(int32_t a, int32_t b, int32_t c1, int32_t c2, uint64_t seed, mf_host_graph **out);
+/* R-MAT quadrant recursion (GTgraph/R-MAT/graph.c:158-182), self-loops and duplicates dropped,
+ * capacities uniform [1, cap_max], s = max out-degree, t = max in-degree. */
+mf_status mf_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                      int32_t cap_max, uint64_t seed, mf_host_graph **out);
+/* Bipartite matching network: |L| left, |R| right, Poisson(mean_deg) distinct arcs per left
+ * vertex, unit capacities; vertex 0 = source, 1..L left, L+1..L+R right, L+R+1 = sink. */
+mf_status mf_gen_bipartite(int32_t L, int32_t R, double mean_deg, uint64_t seed, mf_host_graph **out);
+
+/* ---- host-side residual layouts (ResidualGraph::buildFromCSRGraph) ----
+ * Exposed for layout tests.  Arrays are allocated by the library; free with mf_layout_free.
+ *   RCSR : a = roffsets[n+1], b = rdst[m], c = flow_index[m]    (graph.cpp:317-402)
+ *   BCSR / BCSR2 : a = slot offsets[n+1], b = slot dst[S], c = reverse slot[S],
+ *                  d = initial residual[S], e = forward slot of each arc[m]   (bcsr graph.cpp:322-365) */
+typedef struct mf_layout_arrays {
+  int32_t n;
+  int64_t slots;
+  int32_t *a, *b, *c, *d, *e;
+} mf_layout_arrays;
+mf_status mf_build_layout(const mf_csr *g, mf_layout layout, mf_layout_arrays *out);
+void mf_layout_free(mf_layout_arrays *arr);
+
+/* ---- single-instance device solve ----
+ * mf_graph_upload + mf_graph_solve replace main.cu:109-196 (allocation, preflow, H2D copies,
+ * push_relabel()).  The graph stays resident; every solve restarts from the capacities. */
+mf_status mf_graph_upload(mf_context *ctx, const mf_csr *g, mf_layout layout, mf_graph **out);
+mf_status mf_graph_solve(mf_context *ctx, mf_graph *g, mf_algo algo, const mf_options *opts,
+                         mf_result *res);
+/* Flow on every input arc (cap - residual), m entries, after a solve.  The flow is a maximum
+ * preflow; with return_excess != 0 the excess stranded at vertices that cannot reach t is sent
+ * back to s on the host so the result satisfies conservation everywhere except s and t. */
+mf_status mf_graph_read_flow(mf_context *ctx, mf_graph *g, int32_t *flow_out, int32_t return_excess);
+/* Min-cut side after a solve: side[v] = 1 if v can reach t in the residual graph, else 0. */
+mf_status mf_graph_read_cut(mf_context *ctx, mf_graph *g, uint8_t *sink_side);
+/* Heights and excesses after a solve (n entries each; either may be NULL). */
+mf_status mf_graph_read_state(mf_context *ctx, mf_graph *g, int32_t *height, int32_t *excess);
+mf_status mf_graph_destroy(mf_context *ctx, mf_graph *g);
+
+/* One call: upload, solve, optional flow read-back (flow_out may be NULL), release. */
+mf_status mf_solve(mf_context *ctx, const mf_csr *g, mf_algo algo, mf_layout layout,
+                   const mf_options *opts, mf_result *res, int32_t *flow_out);
+
+/* Drop-in for the reference host->solver interface on RCSR arrays:
+ * preflow() (maxflow-cuda/src/preflow.cu:3-31) followed by push_relabel()
+ * (maxflow-cuda/src/push_relabel.cu:35-221), declared at maxflow-cuda/include/parallel_graph.cuh:44-54.
+ * Same host arrays and meaning: on return h, e, fflows, bflows hold the final state, *excess_total
+ * the total excess that reached the sink, and e[t] the max-flow value.  flow_idx is used as given. */
+mf_status mf_rcsr_push_relabel(mf_context *ctx, int algo_type, int V, int E, int source, int sink,
+                               int *cpu_height, int *cpu_excess_flow,
+                               const int *cpu_offsets, const int *cpu_destinations,
+                               const int *cpu_capacities, int *cpu_fflows, int *cpu_bflows,
+                               const int *cpu_roffsets, const int *cpu_rdestinations,
+                               const int *cpu_flow_idx, int *excess_total,
+                               const mf_options *opts, mf_result *res);
+
+/* ---- batched solve (no reference counterpart) ----
+ * k independent instances packed into one concatenated residual graph with per-instance
+ * offsets; solved by a persistent kernel in which each workgroup pulls whole instances from a
+ * device queue.  Instances larger than opts->reserved[0] vertices (0 = automatic) are solved
+ * by the grid-wide single-instance path inside the same call.  values[k] receives each value. */
+mf_status mf_batch_solve(mf_context *ctx, const mf_csr *inst, int32_t k, const mf_options *opts,
+                         int64_t *values, mf_result *total);
+
+/* ---- serving (no reference counterpart) ----
+ * A request loop over a device-resident arena of hbm_budget_bytes.  Submitted graphs are copied
+ * (the caller's arrays may be freed after submit) and admitted into the arena when their
+ * footprint fits; admitted requests are solved by batched launches that overlap on the device;
+ * finished requests are retired and their memory reused. */
+typedef struct mf_server mf_server;
+typedef struct mf_server_stats {
+  int64_t submitted, admitted, completed, rejected;
+  int64_t launches;
+  int64_t arena_bytes, arena_peak_bytes;
+} mf_server_stats;
+mf_status mf_server_create(mf_context *ctx, int64_t hbm_budget_bytes, int32_t max_inflight_launches,
+                           mf_server **out);
+/* Submit a request; *request_id receives its id.  MF_ERR_BUDGET if it can never fit. */
+mf_status mf_server_submit(mf_server *srv, const mf_csr *g, int64_t *request_id);
+/* Advance the loop: admit what fits, launch, retire finished launches.  Non-blocking unless
+ * wait != 0 (then blocks until at least one request completes or nothing is pending). */
+mf_status mf_server_progress(mf_server *srv, int32_t wait);
+/* Result of a request: MF_ERR_NOT_READY until it has completed. */
+mf_status mf_server_result(mf_server *srv, int64_t request_id, int64_t *value, double *device_ms);
+mf_status mf_server_stats_get(mf_server *srv, mf_server_stats *st);
+mf_status mf_server_destroy(mf_server *srv);
+/* Bytes of device memory a graph occupies once admitted (the admission footprint). */
+int64_t mf_footprint_bytes(int32_t n, int64_t m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MF_H */

@@ -1,0 +1,247 @@
+// Device-side building blocks of the push-relabel solver for gfx950 (wave64).
+//
+// Layout accessors: every kernel is written once against the small interface below and
+// instantiated for the three residual layouts.  A "segment" is a contiguous slot range of a
+// vertex: BCSR and BCSR2 have one segment per vertex (forward and reverse slots mixed), RCSR
+// has two (forward arcs, then reverse arcs).  A "key" names one residual direction and is
+// what a push acts on.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <cstdint>
+
+namespace mfk {
+
+constexpr int kNumClasses = 5;   // degree classes of the worklists
+constexpr int kBlock = 256;      // threads per block of the cycle / BFS-level kernels
+constexpr int kWgBlock = 1024;   // threads of the one-workgroup BFS kernel
+
+__device__ __forceinline__ int ld_acq(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- residual layouts ------------------------------------------------------------------------
+
+// BCSR re-laid for the scan: slot j holds {dst, residual} in one 8-byte word so the
+// neighbour scan is a single coalesced 64-bit load per slot; the reverse slot index is a
+// separate array read only by pushes and the BFS.
+struct LayoutBCSR {
+  static constexpr int kSeg = 1;
+  const int *off;
+  int2 *dc;
+  const int *rev;
+  __device__ __forceinline__ int2 range(int u, int) const { return make_int2(off[u], off[u + 1]); }
+  __device__ __forceinline__ void scan(int, int j, int &w, int &r, int &key) const {
+    int2 x = dc[j];
+    w = x.x;
+    r = x.y;
+    key = j;
+  }
+  __device__ __forceinline__ int target(int key) const { return dc[key].x; }
+  __device__ __forceinline__ int residual(int key) const { return ld_acq(&dc[key].y); }
+  __device__ __forceinline__ void push(int key, int d) const {
+    atomicSub(&dc[key].y, d);
+    atomicAdd(&dc[rev[key]].y, d);
+  }
+  // BFS: slot j of frontier vertex u -> neighbour w and residual of the arc w -> u
+  __device__ __forceinline__ void bfs(int, int j, int &w, int &rin) const {
+    w = dc[j].x;
+    rin = ld_acq(&dc[rev[j]].y);
+  }
+  __device__ __forceinline__ int degree(int u) const { return off[u + 1] - off[u]; }
+};
+
+// BCSR2 (maxflow-bcsr2 layout): dst array plus interleaved {residual, reverse slot} pairs.
+struct LayoutBCSR2 {
+  static constexpr int kSeg = 1;
+  const int *off;
+  const int *dst;
+  int2 *cr;
+  __device__ __forceinline__ int2 range(int u, int) const { return make_int2(off[u], off[u + 1]); }
+  __device__ __forceinline__ void scan(int, int j, int &w, int &r, int &key) const {
+    w = dst[j];
+    r = cr[j].x;
+    key = j;
+  }
+  __device__ __forceinline__ int target(int key) const { return dst[key]; }
+  __device__ __forceinline__ int residual(int key) const { return ld_acq(&cr[key].x); }
+  __device__ __forceinline__ void push(int key, int d) const {
+    const int rv = cr[key].y;
+    atomicSub(&cr[key].x, d);
+    atomicAdd(&cr[rv].x, d);
+  }
+  __device__ __forceinline__ void bfs(int, int j, int &w, int &rin) const {
+    w = dst[j];
+    rin = ld_acq(&cr[cr[j].y].x);
+  }
+  __device__ __forceinline__ int degree(int u) const { return off[u + 1] - off[u]; }
+};
+
+// RCSR (maxflow-cuda layout): forward arcs {dst, forward residual} per arc, backward residual
+// per arc, reverse CSR {tail, arc index} per in-arc.  Key: arc a (< m) for the forward
+// direction, m + k for reverse position k.
+struct LayoutRCSR {
+  static constexpr int kSeg = 2;
+  const int *off;
+  int2 *fdc;
+  int *bcf;
+  const int *roff;
+  const int2 *rdi;
+  int m;
+  __device__ __forceinline__ int2 range(int u, int seg) const {
+    return seg == 0 ? make_int2(off[u], off[u + 1]) : make_int2(roff[u], roff[u + 1]);
+  }
+  __device__ __forceinline__ void scan(int seg, int j, int &w, int &r, int &key) const {
+    if (seg == 0) {
+      int2 x = fdc[j];
+      w = x.x;
+      r = x.y;
+      key = j;
+    } else {
+      int2 y = rdi[j];
+      w = y.x;
+      r = bcf[y.y];
+      key = m + j;
+    }
+  }
+  __device__ __forceinline__ int target(int key) const {
+    return key < m ? fdc[key].x : rdi[key - m].x;
+  }
+  __device__ __forceinline__ int residual(int key) const {
+    return key < m ? ld_acq(&fdc[key].y) : ld_acq(&bcf[rdi[key - m].y]);
+  }
+  __device__ __forceinline__ void push(int key, int d) const {
+    if (key < m) {
+      atomicSub(&fdc[key].y, d);
+      atomicAdd(&bcf[key], d);
+    } else {
+      const int a = rdi[key - m].y;
+      atomicSub(&bcf[a], d);
+      atomicAdd(&fdc[a].y, d);
+    }
+  }
+  __device__ __forceinline__ void bfs(int seg, int j, int &w, int &rin) const {
+    if (seg == 0) {  // forward arc u -> w: the arc w -> u is its backward direction
+      w = fdc[j].x;
+      rin = ld_acq(&bcf[j]);
+    } else {  // in-arc a = w -> u: its forward residual
+      int2 y = rdi[j];
+      w = y.x;
+      rin = ld_acq(&fdc[y.y].y);
+    }
+  }
+  __device__ __forceinline__ int degree(int u) const {
+    return off[u + 1] - off[u] + roff[u + 1] - roff[u];
+  }
+};
+
+// ---- worklists -------------------------------------------------------------------------------
+
+// One list set: per degree class a region of the item array and a counter.  An append past
+// the region's capacity is dropped and raises *ovf (the host turns that into an error), so a
+// logic error can never write outside the allocation.
+struct ListSet {
+  int *items;
+  int cls_off[kNumClasses];
+  int cls_cap[kNumClasses];
+  int *cnt;  // kNumClasses counters
+  int *ovf;
+};
+
+// Number of items of class c, clamped to the region capacity.
+__device__ __forceinline__ int list_count(const ListSet &L, int c) {
+  const int k = ld_acq(&L.cnt[c]);
+  return k < L.cls_cap[c] ? k : L.cls_cap[c];
+}
+
+struct Dispatch {
+  int nclass;
+  int lanes[kNumClasses];
+  int blk_begin[kNumClasses + 1];
+};
+
+// Wave-aggregated append: one atomic per wave and class, lanes get consecutive slots
+// (ballot + popcount of lower lanes).
+__device__ __forceinline__ void append(const ListSet &L, int item, bool want, int cls,
+                                       int nclass) {
+  const int lane = __lane_id();
+  for (int c = 0; c < nclass; ++c) {
+    const bool mine = want && cls == c;
+    const unsigned long long mask = __ballot(mine);
+    if (mask == 0) continue;
+    const int leader = __ffsll((long long)mask) - 1;
+    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&L.cnt[c], __popcll(mask));
+    base = __shfl(base, leader);
+    if (mine) {
+      if (base + rank < L.cls_cap[c]) L.items[L.cls_off[c] + base + rank] = item;
+      else *L.ovf = 1;
+    }
+  }
+}
+
+__device__ __forceinline__ unsigned long long pack_hk(int h, int key) {
+  return ((unsigned long long)(unsigned)h << 32) | (unsigned)key;
+}
+
+template <int T>
+__device__ __forceinline__ unsigned long long tile_min(unsigned long long v) {
+#pragma unroll
+  for (int off = T / 2; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(v, off, T);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+template <int T>
+__device__ __forceinline__ long long tile_sum(long long v) {
+#pragma unroll
+  for (int off = T / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, T);
+  return v;
+}
+
+// Statistics accumulated per thread and flushed once per block.
+struct Stats {
+  long long active, slots, pushes, relabels;
+};
+
+enum StatIdx {
+  ST_ACTIVE = 0,
+  ST_SLOTS = 1,
+  ST_PUSHES = 2,
+  ST_RELABELS = 3,
+  ST_BFS_VERTS = 4,
+  ST_BFS_SLOTS = 5,
+  ST_SOURCE_OUT = 6,
+  ST_COUNT = 8
+};
+
+__device__ __forceinline__ void block_flush(long long a, long long b, long long c, long long d,
+                                            unsigned long long *out, int i0) {
+  __shared__ long long red[4][kWgBlock / 64];
+  a = tile_sum<64>(a);
+  b = tile_sum<64>(b);
+  c = tile_sum<64>(c);
+  d = tile_sum<64>(d);
+  const int w = threadIdx.x / 64;
+  if (__lane_id() == 0) {
+    red[0][w] = a;
+    red[1][w] = b;
+    red[2][w] = c;
+    red[3][w] = d;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    long long s = 0;
+    for (int k = 0; k < (int)(blockDim.x / 64); ++k) s += red[threadIdx.x][k];
+    if (s) atomicAdd(&out[i0 + threadIdx.x], (unsigned long long)s);
+  }
+}
+
+}  // namespace mfk

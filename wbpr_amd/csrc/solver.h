@@ -1,0 +1,36 @@
+// Host-side declarations shared by the solver sources.
+#pragma once
+
+#include <cstdint>
+
+#include "../../include/mf.h"
+
+namespace mf {
+
+// Degree classes of the worklists: class k holds vertices with slot degree <= maxdeg[k] and is
+// processed with lanes[k] lanes per vertex (1 = thread-centric, 256 = whole block).
+struct ClassConfig {
+  int nclass = 1;
+  int lanes[5] = {1, 0, 0, 0, 0};
+  int maxdeg[5] = {0, 0, 0, 0, 0};
+};
+
+ClassConfig class_config(mf_algo algo, int tile_lanes);
+int class_of_degree(const ClassConfig &c, int64_t deg);
+
+int64_t algorithmic_bytes_cycle(mf_layout layout, int64_t active, int64_t slots, int64_t pushes,
+                                int64_t relabels);
+int64_t algorithmic_bytes_bfs(mf_layout layout, int32_t n, int64_t rounds, int64_t verts,
+                              int64_t slots);
+
+const char *last_device_error();
+
+}  // namespace mf
+
+struct mf_context;
+namespace mf {
+// Solves k instances packed into one CSR (vertex ranges disjoint, src/snk global ids).
+mf_status batch_solve_packed(mf_context *ctx, const mf_csr *packed, const int32_t *src,
+                             const int32_t *snk, int32_t k, mf_algo algo, mf_layout layout,
+                             const mf_options &o, int64_t *values, mf_result *res);
+}  // namespace mf

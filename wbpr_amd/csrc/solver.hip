@@ -1,0 +1,1177 @@
+// Push-relabel solver on the GPU: kernels and the host loop that drives them.
+//
+// The device graph may hold k independent instances packed into one vertex and slot space
+// (k = 1 for a single solve); every kernel below works on all instances at once.  Heights use
+// three reserved values: sinks 0, vertices not reached by the last global relabel H = N
+// (inactive), sources H + 1.  Active vertex: excess > 0 and height < H.
+//
+// One round = one global relabel followed by K push-relabel cycles:
+//   preflow   saturate every residual arc out of every source  (maxflow-cuda/src/preflow.cu:3-31)
+//   relabel   BFS from all sinks over reverse residual arcs: h(v) = dist(v, sink of v), reached
+//             vertices holding excess form the first worklist.  If no reached vertex holds
+//             excess, e(t_i) is the max-flow value of every instance: all arcs from the
+//             unreached side into the reached side are saturated.
+//             (replaces the CPU relabel maxflow-cuda/src/global_relabel.cu:4-178 and the GPU one
+//             of maxflow-bcsr2/src/global_relabel_gpu.cu:4-190)
+//   cycle     every active vertex u finds its lowest residual neighbour w; if h(u) > h(w) it
+//             pushes min(e(u), r(u,w)) with integer atomics, else relabels h(u) = h(w) + 1
+//             (TC: maxflow-cuda/src/push_relabel_kernel.cu:14-158, one lane per vertex;
+//              VC: :253-537, a tile of 8..256 lanes with a min-reduction per vertex).
+//             Vertices that keep excess and vertices that receive their first excess are
+//             appended to the next worklist exactly once (see finish_vertex).
+// The push/relabel rule is the lock-free one of He & Hong that the reference uses: only the
+// owner of u lowers e(u) and r(u,*), so stale reads of other vertices' state are safe.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "kernels.h"
+#include "mf_internal.h"
+#include "solver.h"
+
+using namespace mfk;
+
+namespace mf {
+
+// ============================================================================================
+// kernels
+// ============================================================================================
+
+struct CycleArgs {
+  int *h;
+  int *e;
+  int H;  // unreached / inactive height (= number of vertices)
+  const unsigned char *vcls;
+  int nclass;
+  ListSet in, out;
+  int *clear_cnt;
+  unsigned long long *stats;
+};
+
+// Push or relabel u given the packed (height, key) minimum over its residual arcs, then append
+// u and/or the push target to the next worklist.  Exactly-once argument: a vertex is in the
+// current list iff its excess became positive since it was last processed.  The owner re-adds
+// u iff its own atomicSub leaves a positive remainder (or it relabelled below H); a pusher adds
+// w iff its atomicAdd found e(w) == 0.  Whichever of the two atomics on e(u) comes second sees
+// the other's effect, so exactly one of them appends.
+template <class L>
+__device__ __forceinline__ void finish_vertex(const L &g, const CycleArgs &a, int u, int hu,
+                                              unsigned long long best, bool leader, int deg,
+                                              Stats &st) {
+  bool want_u = false, want_w = false;
+  int w = 0;
+  if (leader) {
+    st.active++;
+    st.slots += deg;
+    if (best == ~0ull) {  // no residual arc left: u can never reach a sink again
+      a.h[u] = a.H;
+      st.relabels++;
+    } else {
+      const int hmin = (int)(best >> 32);
+      const int key = (int)(unsigned)(best & 0xffffffffull);
+      if (hu > hmin) {
+        w = g.target(key);
+        const int eu = ld_acq(&a.e[u]);
+        const int rr = g.residual(key);
+        const int d = min(eu, rr);
+        if (d > 0) {
+          g.push(key, d);
+          const int old_w = atomicAdd(&a.e[w], d);
+          const int rem = atomicSub(&a.e[u], d) - d;
+          want_w = old_w == 0 && hmin > 0;  // height 0 only at sinks
+          want_u = rem > 0;
+          st.pushes++;
+        } else {
+          want_u = eu > 0;
+        }
+      } else {
+        const int nh = min(hmin + 1, a.H);
+        a.h[u] = nh;
+        st.relabels++;
+        want_u = nh < a.H;
+      }
+    }
+  }
+  const int cu = a.vcls ? a.vcls[u] : 0;
+  append(a.out, u, want_u, cu, a.nclass);
+  const int cw = (want_w && a.vcls) ? a.vcls[w] : 0;
+  append(a.out, w, want_w, cw, a.nclass);
+}
+
+// One discharge step of u by a tile of T lanes (T <= 64, inside one wave).
+template <int T, class L>
+__device__ __forceinline__ void discharge_tile(const L &g, const CycleArgs &a, int u, int lane,
+                                               Stats &st) {
+  const int hu = a.h[u];
+  if (hu >= a.H) return;  // became inactive after it was queued (all lanes read the same value)
+  unsigned long long best = ~0ull;
+  int deg = 0;
+#pragma unroll
+  for (int seg = 0; seg < L::kSeg; ++seg) {
+    const int2 r = g.range(u, seg);
+    deg += r.y - r.x;
+    for (int j = r.x + lane; j < r.y; j += T) {
+      int w, res, key;
+      g.scan(seg, j, w, res, key);
+      if (res > 0 && w != u) {
+        const unsigned long long k = pack_hk(a.h[w], key);
+        best = k < best ? k : best;
+      }
+    }
+  }
+  best = tile_min<T>(best);
+  finish_vertex(g, a, u, hu, best, lane == 0, deg, st);
+}
+
+// One discharge step of a hub vertex by the whole block (kBlock lanes, LDS reduction).
+template <class L>
+__device__ __forceinline__ void discharge_block(const L &g, const CycleArgs &a, int u, Stats &st) {
+  __shared__ unsigned long long red[kBlock / 64];
+  const int hu = a.h[u];
+  if (hu >= a.H) return;  // uniform over the block: only this block writes h[u]
+  unsigned long long best = ~0ull;
+  int deg = 0;
+#pragma unroll
+  for (int seg = 0; seg < L::kSeg; ++seg) {
+    const int2 r = g.range(u, seg);
+    deg += r.y - r.x;
+    for (int j = r.x + (int)threadIdx.x; j < r.y; j += kBlock) {
+      int w, res, key;
+      g.scan(seg, j, w, res, key);
+      if (res > 0 && w != u) {
+        const unsigned long long k = pack_hk(a.h[w], key);
+        best = k < best ? k : best;
+      }
+    }
+  }
+  best = tile_min<64>(best);
+  if (__lane_id() == 0) red[threadIdx.x / 64] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBlock / 64; ++k) best = red[k] < best ? red[k] : best;
+  }
+  __syncthreads();
+  finish_vertex(g, a, u, hu, best, threadIdx.x == 0, deg, st);
+}
+
+template <int T, class L>
+__device__ __forceinline__ void cycle_class(const L &g, const CycleArgs &a, int c, int cnt, int b,
+                                            int nblk, Stats &st) {
+  const int base = a.in.cls_off[c];
+  if constexpr (T <= 64) {
+    constexpr int tpb = kBlock / T;
+    const int tile = b * tpb + (int)threadIdx.x / T;
+    const int ntiles = nblk * tpb;
+    const int lane = (int)threadIdx.x % T;
+    for (int i = tile; i < cnt; i += ntiles) discharge_tile<T>(g, a, a.in.items[base + i], lane, st);
+  } else {
+    for (int i = b; i < cnt; i += nblk) discharge_block(g, a, a.in.items[base + i], st);
+  }
+}
+
+// Block b of the grid serves degree class c when d.blk_begin[c] <= b < d.blk_begin[c+1].
+__device__ __forceinline__ int block_class(const Dispatch &d) {
+  int c = 0;
+  while (c + 1 < d.nclass && (int)blockIdx.x >= d.blk_begin[c + 1]) ++c;
+  return c;
+}
+
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_cycle(L g, CycleArgs a, Dispatch d) {
+  // counters of the list two cycles ahead: last read by the previous launch, next written by
+  // the following one
+  if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses) a.clear_cnt[threadIdx.x] = 0;
+  const int c = block_class(d);
+  const int nblk = d.blk_begin[c + 1] - d.blk_begin[c];
+  const int b = (int)blockIdx.x - d.blk_begin[c];
+  const int cnt = list_count(a.in, c);
+  Stats st{0, 0, 0, 0};
+  switch (d.lanes[c]) {
+    case 1: cycle_class<1>(g, a, c, cnt, b, nblk, st); break;
+    case 8: cycle_class<8>(g, a, c, cnt, b, nblk, st); break;
+    case 16: cycle_class<16>(g, a, c, cnt, b, nblk, st); break;
+    case 32: cycle_class<32>(g, a, c, cnt, b, nblk, st); break;
+    case 64: cycle_class<64>(g, a, c, cnt, b, nblk, st); break;
+    default: cycle_class<kBlock>(g, a, c, cnt, b, nblk, st); break;
+  }
+  block_flush(st.active, st.slots, st.pushes, st.relabels, a.stats, ST_ACTIVE);
+}
+
+// ---- global relabel ------------------------------------------------------------------------
+
+struct BfsArgs {
+  int *h;
+  const int *e;
+  int H;
+  const unsigned char *vcls;
+  int nclass;
+  ListSet fr[2];   // frontier items, double-buffered by level parity
+  int *fcnt[3];    // frontier counters: level L reads fcnt[L%3], writes fcnt[(L+1)%3]
+  ListSet active;  // reached vertices with excess (first worklist of the cycles)
+  int *flags;      // [0] a source was reached, [1] a worklist overflowed
+  int *state;      // [0] level, [1] frontier size (exit state of the one-workgroup kernel)
+  unsigned long long *stats;
+};
+
+template <int T, class L>
+__device__ __forceinline__ void bfs_vertex(const L &g, const BfsArgs &a, const ListSet &out, int u,
+                                           int lane, int level, long long &ns) {
+#pragma unroll
+  for (int seg = 0; seg < L::kSeg; ++seg) {
+    const int2 r = g.range(u, seg);
+    if (lane == 0) ns += r.y - r.x;
+    for (int j = r.x + lane; j < r.y; j += T) {
+      int w, rin;
+      g.bfs(seg, j, w, rin);
+      bool claim = false, act = false;
+      if (rin > 0 && w != u) {
+        const int hw = ld_acq(&a.h[w]);
+        if (hw == a.H + 1) {
+          a.flags[0] = 1;  // residual path from a source: preflow invariant broken
+        } else if (hw == a.H && atomicCAS(&a.h[w], a.H, level + 1) == a.H) {
+          claim = true;
+          act = ld_acq(&a.e[w]) > 0;
+        }
+      }
+      const int cw = (claim && a.vcls) ? a.vcls[w] : 0;
+      append(out, w, claim, cw, a.nclass);
+      append(a.active, w, act, cw, a.nclass);
+    }
+  }
+}
+
+template <int T, class L>
+__device__ __forceinline__ void bfs_class(const L &g, const BfsArgs &a, const ListSet &in,
+                                          const ListSet &out, int c, int cnt, int tile, int ntiles,
+                                          int lane, int level, long long &nv, long long &ns) {
+  const int base = in.cls_off[c];
+  for (int i = tile; i < cnt; i += ntiles) {
+    bfs_vertex<T>(g, a, out, in.items[base + i], lane, level, ns);
+    if (lane == 0) ++nv;
+  }
+}
+
+template <class L>
+__device__ __forceinline__ void bfs_dispatch(const L &g, const BfsArgs &a, const ListSet &in,
+                                             const ListSet &out, int c, int lanes, int cnt, int tid,
+                                             int nthreads, int level, long long &nv, long long &ns) {
+  switch (lanes) {
+#define MF_BFS_CASE(TT)                                                                       \
+  case TT:                                                                                    \
+    bfs_class<TT>(g, a, in, out, c, cnt, tid / TT, nthreads / TT, tid % TT, level, nv, ns); \
+    break;
+    MF_BFS_CASE(1)
+    MF_BFS_CASE(8)
+    MF_BFS_CASE(16)
+    MF_BFS_CASE(32)
+    MF_BFS_CASE(64)
+    default:
+      bfs_class<kBlock>(g, a, in, out, c, cnt, tid / kBlock, nthreads / kBlock, tid % kBlock,
+                        level, nv, ns);
+      break;
+#undef MF_BFS_CASE
+  }
+}
+
+// Heights to "unreached", counters to zero.
+__global__ void k_bfs_init(BfsArgs a, int *ccnt0, int *ccnt1) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < a.H; v += stride) a.h[v] = a.H;
+  if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses) {
+    const int c = threadIdx.x;
+    a.fcnt[0][c] = 0;
+    a.fcnt[1][c] = 0;
+    a.fcnt[2][c] = 0;
+    ccnt0[c] = 0;
+    ccnt1[c] = 0;
+    if (c == 0) a.flags[0] = 0;
+  }
+}
+
+// Terminal heights and the level-0 frontier (all sinks).
+__global__ void k_bfs_seed(BfsArgs a, const int *src, const int *snk, int k) {
+  ListSet f0 = a.fr[0];
+  f0.cnt = a.fcnt[0];
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride) {
+    const int t = snk[i];
+    a.h[t] = 0;
+    a.h[src[i]] = a.H + 1;
+    append(f0, t, true, a.vcls ? a.vcls[t] : 0, a.nclass);
+  }
+}
+
+// One BFS level over the whole grid.
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_bfs_level(L g, BfsArgs a, Dispatch d, int level) {
+  if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses)
+    a.fcnt[(level + 2) % 3][threadIdx.x] = 0;
+  const int c = block_class(d);
+  const int nblk = d.blk_begin[c + 1] - d.blk_begin[c];
+  const int b = (int)blockIdx.x - d.blk_begin[c];
+  ListSet in = a.fr[level & 1];
+  in.cnt = a.fcnt[level % 3];
+  ListSet out = a.fr[(level + 1) & 1];
+  out.cnt = a.fcnt[(level + 1) % 3];
+  const int cnt = list_count(in, c);
+  long long nv = 0, ns = 0;
+  bfs_dispatch(g, a, in, out, c, d.lanes[c], cnt, b * kBlock + (int)threadIdx.x, nblk * kBlock,
+               level, nv, ns);
+  block_flush(nv, ns, 0, 0, a.stats, ST_BFS_VERTS);
+}
+
+// All BFS levels inside one workgroup while the frontier holds at most fmax vertices: workgroup
+// barriers replace kernel launches, which is what long-diameter graphs need.
+template <class L>
+__global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d, int level,
+                                                     int fmax) {
+  __shared__ int s_size;
+  long long nv = 0, ns = 0;
+  const int tid = (int)threadIdx.x;
+  while (true) {
+    if (tid == 0) {
+      int *clr = a.fcnt[(level + 2) % 3];
+      for (int c = 0; c < kNumClasses; ++c) atomicExch(&clr[c], 0);
+      int sz = 0;
+      for (int c = 0; c < d.nclass; ++c) sz += ld_acq(&a.fcnt[level % 3][c]);
+      s_size = sz;
+      __threadfence();
+    }
+    __syncthreads();
+    const int size = s_size;
+    if (size == 0 || size > fmax) break;
+    ListSet in = a.fr[level & 1];
+    in.cnt = a.fcnt[level % 3];
+    ListSet out = a.fr[(level + 1) & 1];
+    out.cnt = a.fcnt[(level + 1) % 3];
+    for (int c = 0; c < d.nclass; ++c) {
+      const int cnt = list_count(in, c);
+      if (cnt == 0) continue;
+      bfs_dispatch(g, a, in, out, c, d.lanes[c], cnt, tid, kWgBlock, level, nv, ns);
+    }
+    __threadfence();
+    __syncthreads();
+    ++level;
+  }
+  if (tid == 0) {
+    a.state[0] = level;
+    a.state[1] = s_size;
+  }
+  block_flush(nv, ns, 0, 0, a.stats, ST_BFS_VERTS);
+}
+
+// ---- reset, preflow, results ---------------------------------------------------------------
+
+__global__ void k_reset_pairs(int2 *pairs, const int *cap0, long long count, int field) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < count; j += stride) {
+    if (field == 0) pairs[j].x = cap0[j];
+    else pairs[j].y = cap0[j];
+  }
+}
+
+__global__ void k_reset_vertices(int *h, int *e, int n, int *bcf, long long m) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += stride) {
+    h[v] = 0;
+    e[v] = 0;
+  }
+  if (bcf)
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < m; j += stride)
+      bcf[j] = 0;
+}
+
+// preflow (maxflow-cuda/src/preflow.cu:3-31), one block per source: saturate every residual arc
+// out of the source.  Parallel s -> v arcs add to e(v) (the reference overwrites it).
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_preflow(L g, int *e, const int *src, int k,
+                                                    unsigned long long *stats) {
+  long long sum = 0;
+  for (int i = blockIdx.x; i < k; i += gridDim.x) {
+    const int s = src[i];
+#pragma unroll
+    for (int seg = 0; seg < L::kSeg; ++seg) {
+      const int2 r = g.range(s, seg);
+      for (int j = r.x + (int)threadIdx.x; j < r.y; j += kBlock) {
+        int w, res, key;
+        g.scan(seg, j, w, res, key);
+        if (res > 0 && w != s) {
+          g.push(key, res);
+          atomicAdd(&e[w], res);
+          sum += res;
+        }
+      }
+    }
+  }
+  block_flush(sum, 0, 0, 0, stats, ST_SOURCE_OUT);
+}
+
+__global__ void k_gather_values(const int *e, const int *snk, int k, long long *out) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride) out[i] = e[snk[i]];
+}
+
+}  // namespace mf
+
+// ============================================================================================
+// host side
+// ============================================================================================
+namespace mf {
+
+static thread_local char g_last_error[256];
+
+const char *last_device_error() { return g_last_error; }
+
+#define HIP_TRY(expr)                                                                         \
+  do {                                                                                        \
+    hipError_t _e = (expr);                                                                   \
+    if (_e != hipSuccess) {                                                                   \
+      snprintf(g_last_error, sizeof(g_last_error), "%s:%d %s", __FILE__, __LINE__,            \
+               hipGetErrorString(_e));                                                        \
+      return _e == hipErrorOutOfMemory ? MF_ERR_OUT_OF_MEMORY : MF_ERR_DEVICE;                \
+    }                                                                                         \
+  } while (0)
+
+static double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+template <class T>
+static mf_status dmalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  HIP_TRY(hipMalloc((void **)p, sizeof(T) * count));
+  return MF_OK;
+}
+
+ClassConfig class_config(mf_algo algo, int tile_lanes) {
+  ClassConfig c;
+  if (algo == MF_ALGO_TC) {
+    c.nclass = 1;
+    c.lanes[0] = 1;
+    c.maxdeg[0] = INT32_MAX;
+  } else if (tile_lanes > 0) {
+    c.nclass = 1;
+    c.lanes[0] = tile_lanes;
+    c.maxdeg[0] = INT32_MAX;
+  } else {
+    c.nclass = 5;
+    const int lanes[5] = {8, 16, 32, 64, kBlock};
+    const int maxdeg[5] = {8, 16, 32, 1024, INT32_MAX};
+    for (int k = 0; k < 5; ++k) {
+      c.lanes[k] = lanes[k];
+      c.maxdeg[k] = maxdeg[k];
+    }
+  }
+  return c;
+}
+
+int class_of_degree(const ClassConfig &c, int64_t deg) {
+  for (int k = 0; k < c.nclass; ++k)
+    if (deg <= c.maxdeg[k]) return k;
+  return c.nclass - 1;
+}
+
+static Dispatch make_dispatch(const ClassConfig &c, const int64_t *pop, int num_cu) {
+  Dispatch d;
+  memset(&d, 0, sizeof(d));
+  d.nclass = c.nclass;
+  int b = 0;
+  for (int k = 0; k < c.nclass; ++k) {
+    d.lanes[k] = c.lanes[k];
+    d.blk_begin[k] = b;
+    if (pop[k] == 0) continue;
+    int64_t nb;
+    if (c.lanes[k] <= 64) {
+      const int64_t tpb = kBlock / c.lanes[k];
+      nb = std::min<int64_t>((pop[k] + tpb - 1) / tpb, (int64_t)8 * num_cu);
+    } else {
+      nb = std::min<int64_t>(pop[k], (int64_t)2 * num_cu);
+    }
+    b += (int)std::max<int64_t>(nb, 1);
+  }
+  for (int k = c.nclass; k <= kNumClasses; ++k) d.blk_begin[k] = b;
+  return d;
+}
+
+int64_t algorithmic_bytes_cycle(mf_layout layout, int64_t active, int64_t slots, int64_t pushes,
+                                int64_t relabels) {
+  // per active vertex: worklist item 4 + offsets 8 (RCSR 16) + own height 4 + append 4
+  // per slot: slot record (BCSR 8, BCSR2 12, RCSR 8 forward / 12 reverse -> 10) + height 4
+  // per push: excess 4 + residual re-read 4 + four int32 read-modify-writes 32
+  // per relabel: height store 4
+  const int64_t per_v = layout == MF_LAYOUT_RCSR ? 28 : 20;
+  const int64_t per_slot = layout == MF_LAYOUT_BCSR ? 12 : (layout == MF_LAYOUT_BCSR2 ? 16 : 14);
+  return active * per_v + slots * per_slot + pushes * 40 + relabels * 4;
+}
+
+int64_t algorithmic_bytes_bfs(mf_layout layout, int32_t n, int64_t rounds, int64_t verts,
+                              int64_t slots) {
+  // per round: height initialisation 4n; per frontier vertex: item 4 + offsets 8;
+  // per slot: slot record + reverse residual 4 + neighbour height 4
+  const int64_t per_slot = layout == MF_LAYOUT_BCSR ? 16 : (layout == MF_LAYOUT_BCSR2 ? 20 : 14);
+  return rounds * 4 * (int64_t)n + verts * 12 + slots * per_slot;
+}
+
+}  // namespace mf
+
+struct mf_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cu = 0;
+  char name[256] = {0};
+  char arch[64] = {0};
+  std::vector<hipEvent_t> prof_ev;
+};
+
+namespace mf {
+
+// Device-resident residual graph of k packed instances.
+struct DevGraph {
+  mf_layout layout = MF_LAYOUT_BCSR;
+  int32_t n = 0;  // vertices of all instances
+  int64_t m = 0, S = 0;
+  int32_t k = 0;
+  // layout arrays
+  int *off = nullptr;
+  int2 *dc = nullptr;   // BCSR {dst, residual}
+  int *rev = nullptr;   // BCSR reverse slot
+  int *dst = nullptr;   // BCSR2 dst
+  int2 *cr = nullptr;   // BCSR2 {residual, reverse slot}
+  int2 *fdc = nullptr;  // RCSR {dst, forward residual}
+  int *bcf = nullptr;   // RCSR backward residual
+  int *roff = nullptr;  // RCSR reverse offsets
+  int2 *rdi = nullptr;  // RCSR {tail, arc}
+  int *cap0 = nullptr;  // initial residual per slot (per arc for RCSR)
+  // terminals
+  int *src = nullptr, *snk = nullptr;
+  long long *values = nullptr;
+  // state
+  int *h = nullptr, *e = nullptr;
+  int *lists = nullptr;  // 4 sets x n items
+  int *ctl = nullptr;    // counters and flags
+  unsigned long long *stats = nullptr;
+  unsigned char *vcls = nullptr;  // VC degree class per vertex
+  int64_t vc_pop[kNumClasses] = {0};
+  // host copies for result read-back
+  std::vector<int32_t> h_off, h_dst, h_cap, h_fslot, h_src, h_snk;
+  bool solved = false;
+
+  LayoutBCSR bcsr() const { return LayoutBCSR{off, dc, rev}; }
+  LayoutBCSR2 bcsr2() const { return LayoutBCSR2{off, dst, cr}; }
+  LayoutRCSR rcsr() const { return LayoutRCSR{off, fdc, bcf, roff, rdi, (int)m}; }
+};
+
+// control block (int offsets)
+enum {
+  CTL_FCNT = 0,                           // 3 x kNumClasses frontier counters
+  CTL_CCNT = CTL_FCNT + 3 * kNumClasses,  // 3 x kNumClasses worklist counters
+  CTL_FLAGS = CTL_CCNT + 3 * kNumClasses, // [0] source reached, [1] list overflow
+  CTL_STATE = CTL_FLAGS + 2,              // [0] level, [1] frontier size
+  CTL_SIZE = CTL_STATE + 4
+};
+
+static void free_dev(DevGraph *g) {
+  void *ptrs[] = {g->off, g->dc,  g->rev, g->dst,    g->cr,    g->fdc,   g->bcf,
+                  g->roff, g->rdi, g->cap0, g->src,   g->snk,   g->values, g->h,
+                  g->e,   g->lists, g->ctl, g->stats, g->vcls};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+}
+
+// Uploads a (packed) graph: c is the union CSR, src/snk the k terminal pairs.
+static mf_status upload(mf_context *ctx, const mf_csr *c, const int32_t *src, const int32_t *snk,
+                        int32_t k, mf_layout layout, DevGraph &G) {
+  G.layout = layout;
+  G.n = c->n;
+  G.m = c->m;
+  G.k = k;
+  G.h_off.assign(c->offsets, c->offsets + c->n + 1);
+  G.h_dst.assign(c->dst, c->dst + c->m);
+  G.h_cap.resize((size_t)c->m);
+  for (int64_t i = 0; i < c->m; ++i) G.h_cap[(size_t)i] = c->cap ? c->cap[i] : 1;
+  G.h_src.assign(src, src + k);
+  G.h_snk.assign(snk, snk + k);
+  if ((int64_t)c->n + 1 >= INT32_MAX) return MF_ERR_OVERFLOW;
+  for (int32_t i = 0; i < k; ++i) {
+    int64_t sc = 0;
+    for (int32_t a = c->offsets[src[i]]; a < c->offsets[src[i] + 1]; ++a)
+      if (c->dst[a] != src[i]) sc += G.h_cap[(size_t)a];
+    if (sc > INT32_MAX) return MF_ERR_OVERFLOW;  // every excess must fit int32
+  }
+  hipStream_t st = ctx->stream;
+  std::vector<int64_t> degree((size_t)c->n, 0);
+  mf_status rc;
+  if (layout == MF_LAYOUT_RCSR) {
+    RcsrLayout R;
+    if ((rc = build_rcsr_layout(c, R)) != MF_OK) return rc;
+    G.S = c->m;
+    std::vector<int2> fdc((size_t)c->m), rdi((size_t)c->m);
+    for (int64_t i = 0; i < c->m; ++i) fdc[(size_t)i] = make_int2(c->dst[i], G.h_cap[(size_t)i]);
+    for (int64_t q = 0; q < c->m; ++q)
+      rdi[(size_t)q] = make_int2(R.rdst[(size_t)q], R.fidx[(size_t)q]);
+    if ((rc = dmalloc(&G.off, (size_t)c->n + 1)) || (rc = dmalloc(&G.fdc, (size_t)c->m)) ||
+        (rc = dmalloc(&G.bcf, (size_t)c->m)) || (rc = dmalloc(&G.roff, (size_t)c->n + 1)) ||
+        (rc = dmalloc(&G.rdi, (size_t)c->m)) || (rc = dmalloc(&G.cap0, (size_t)c->m)))
+      return rc;
+    HIP_TRY(hipMemcpyAsync(G.off, c->offsets, sizeof(int) * (c->n + 1), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(G.fdc, fdc.data(), sizeof(int2) * c->m, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(G.roff, R.roff.data(), sizeof(int) * (c->n + 1), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(G.rdi, rdi.data(), sizeof(int2) * c->m, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(G.cap0, G.h_cap.data(), sizeof(int) * c->m, hipMemcpyHostToDevice, st));
+    for (int32_t v = 0; v < c->n; ++v)
+      degree[(size_t)v] = (int64_t)(c->offsets[v + 1] - c->offsets[v]) +
+                          (R.roff[(size_t)v + 1] - R.roff[(size_t)v]);
+    HIP_TRY(hipStreamSynchronize(st));
+  } else {
+    SlotLayout B;
+    if ((rc = build_slot_layout(c, B)) != MF_OK) return rc;
+    G.S = B.slots;
+    G.h_fslot = std::move(B.fslot);
+    if ((rc = dmalloc(&G.off, (size_t)c->n + 1)) || (rc = dmalloc(&G.cap0, (size_t)G.S))) return rc;
+    HIP_TRY(hipMemcpyAsync(G.off, B.off.data(), sizeof(int) * (c->n + 1), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(G.cap0, B.cap0.data(), sizeof(int) * G.S, hipMemcpyHostToDevice, st));
+    if (layout == MF_LAYOUT_BCSR) {
+      std::vector<int2> dc((size_t)G.S);
+      for (int64_t j = 0; j < G.S; ++j) dc[(size_t)j] = make_int2(B.dst[(size_t)j], B.cap0[(size_t)j]);
+      if ((rc = dmalloc(&G.dc, (size_t)G.S)) || (rc = dmalloc(&G.rev, (size_t)G.S))) return rc;
+      HIP_TRY(hipMemcpyAsync(G.dc, dc.data(), sizeof(int2) * G.S, hipMemcpyHostToDevice, st));
+      HIP_TRY(hipMemcpyAsync(G.rev, B.rev.data(), sizeof(int) * G.S, hipMemcpyHostToDevice, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    } else {
+      std::vector<int2> cr((size_t)G.S);
+      for (int64_t j = 0; j < G.S; ++j) cr[(size_t)j] = make_int2(B.cap0[(size_t)j], B.rev[(size_t)j]);
+      if ((rc = dmalloc(&G.dst, (size_t)G.S)) || (rc = dmalloc(&G.cr, (size_t)G.S))) return rc;
+      HIP_TRY(hipMemcpyAsync(G.dst, B.dst.data(), sizeof(int) * G.S, hipMemcpyHostToDevice, st));
+      HIP_TRY(hipMemcpyAsync(G.cr, cr.data(), sizeof(int2) * G.S, hipMemcpyHostToDevice, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
+    for (int32_t v = 0; v < c->n; ++v) degree[(size_t)v] = B.off[(size_t)v + 1] - B.off[(size_t)v];
+  }
+  if ((rc = dmalloc(&G.h, (size_t)c->n)) || (rc = dmalloc(&G.e, (size_t)c->n)) ||
+      (rc = dmalloc(&G.lists, (size_t)4 * c->n)) || (rc = dmalloc(&G.ctl, (size_t)CTL_SIZE)) ||
+      (rc = dmalloc(&G.stats, (size_t)16)) || (rc = dmalloc(&G.vcls, (size_t)c->n)) ||
+      (rc = dmalloc(&G.src, (size_t)k)) || (rc = dmalloc(&G.snk, (size_t)k)) ||
+      (rc = dmalloc(&G.values, (size_t)k)))
+    return rc;
+  // VC degree classes
+  const ClassConfig vc = class_config(MF_ALGO_VC, 0);
+  std::vector<unsigned char> cls((size_t)c->n);
+  for (int q = 0; q < kNumClasses; ++q) G.vc_pop[q] = 0;
+  for (int32_t v = 0; v < c->n; ++v) {
+    const int q = class_of_degree(vc, degree[(size_t)v]);
+    cls[(size_t)v] = (unsigned char)q;
+    G.vc_pop[q]++;
+  }
+  HIP_TRY(hipMemcpyAsync(G.vcls, cls.data(), c->n, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(G.src, src, sizeof(int) * k, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(G.snk, snk, sizeof(int) * k, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemsetAsync(G.ctl, 0, sizeof(int) * CTL_SIZE, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return MF_OK;
+}
+
+template <class L>
+static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo algo,
+                           const mf_options &o, mf_result *res) {
+  hipStream_t st = ctx->stream;
+  const int n = G.n;
+  const ClassConfig cfg = class_config(algo, o.tile_lanes);
+  int64_t pop[kNumClasses] = {0};
+  if (cfg.nclass == 1) pop[0] = n;
+  else
+    for (int q = 0; q < kNumClasses; ++q) pop[q] = G.vc_pop[q];
+  const unsigned char *vcls = cfg.nclass == 1 ? nullptr : G.vcls;
+  const Dispatch disp = make_dispatch(cfg, pop, ctx->num_cu);
+  const int grid = disp.blk_begin[cfg.nclass];
+
+  // list sets: F0, F1 (BFS frontier), S0, S1 (cycle worklists); class regions by population
+  ListSet sets[4];
+  for (int q = 0; q < 4; ++q) {
+    sets[q].items = G.lists + (size_t)q * n;
+    int64_t acc = 0;
+    for (int c = 0; c < kNumClasses; ++c) {
+      sets[q].cls_off[c] = (int)acc;
+      sets[q].cls_cap[c] = c < cfg.nclass ? (int)pop[c] : 0;
+      if (c < cfg.nclass) acc += pop[c];
+    }
+    sets[q].cnt = nullptr;
+    sets[q].ovf = G.ctl + CTL_FLAGS + 1;
+  }
+  int *fcnt[3], *ccnt[3];
+  for (int q = 0; q < 3; ++q) {
+    fcnt[q] = G.ctl + CTL_FCNT + q * kNumClasses;
+    ccnt[q] = G.ctl + CTL_CCNT + q * kNumClasses;
+  }
+
+  BfsArgs ba;
+  ba.h = G.h;
+  ba.e = G.e;
+  ba.H = n;
+  ba.vcls = vcls;
+  ba.nclass = cfg.nclass;
+  ba.fr[0] = sets[0];
+  ba.fr[1] = sets[1];
+  for (int q = 0; q < 3; ++q) ba.fcnt[q] = fcnt[q];
+  ba.active = sets[2];
+  ba.active.cnt = ccnt[0];
+  ba.flags = G.ctl + CTL_FLAGS;
+  ba.state = G.ctl + CTL_STATE;
+  ba.stats = G.stats;
+
+  CycleArgs ca;
+  ca.h = G.h;
+  ca.e = G.e;
+  ca.H = n;
+  ca.vcls = vcls;
+  ca.nclass = cfg.nclass;
+  ca.stats = G.stats;
+
+  const int fmax = o.bfs_block_frontier > 0 ? o.bfs_block_frontier : 512;
+  const int fmax_low = std::max(1, fmax / 2);
+  const int K = o.cycles_per_relabel > 0 ? o.cycles_per_relabel : 64;
+  const int64_t max_rounds = o.max_rounds > 0 ? o.max_rounds : (int64_t)100000000;
+  const int vgrid = std::max(1, std::min((n + 255) / 256, 16 * ctx->num_cu));
+  const int kgrid = std::max(1, std::min((G.k + 255) / 256, 16 * ctx->num_cu));
+
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  double kernel_ms = 0, relabel_ms = 0, prof_ms = 0;
+  int64_t prof_launches = 0, cycles = 0, rounds = 0;
+  mf_status status = MF_OK;
+  if (o.profile_kernels && ctx->prof_ev.size() < (size_t)2 * K) {
+    const size_t old = ctx->prof_ev.size();
+    ctx->prof_ev.resize((size_t)2 * K);
+    for (size_t q = old; q < ctx->prof_ev.size(); ++q) HIP_TRY(hipEventCreate(&ctx->prof_ev[q]));
+  }
+
+  const double t0 = now_ms();
+  const int rgrid = std::max(1, (int)std::min<int64_t>((G.S + 255) / 256, 16 * ctx->num_cu));
+  if (G.layout == MF_LAYOUT_BCSR) k_reset_pairs<<<rgrid, 256, 0, st>>>(G.dc, G.cap0, G.S, 1);
+  else if (G.layout == MF_LAYOUT_BCSR2) k_reset_pairs<<<rgrid, 256, 0, st>>>(G.cr, G.cap0, G.S, 0);
+  else k_reset_pairs<<<rgrid, 256, 0, st>>>(G.fdc, G.cap0, G.m, 1);
+  k_reset_vertices<<<rgrid, 256, 0, st>>>(G.h, G.e, n, G.layout == MF_LAYOUT_RCSR ? G.bcf : nullptr,
+                                          G.m);
+  HIP_TRY(hipMemsetAsync(G.stats, 0, sizeof(unsigned long long) * 16, st));
+  HIP_TRY(hipMemsetAsync(G.ctl, 0, sizeof(int) * CTL_SIZE, st));
+  k_preflow<L><<<std::min(G.k, 16 * ctx->num_cu), kBlock, 0, st>>>(lay, G.e, G.src, G.k, G.stats);
+  HIP_TRY(hipGetLastError());
+
+  int hstate[CTL_SIZE];
+  for (;;) {
+    if (rounds >= max_rounds) {
+      status = MF_ERR_ITERATION_LIMIT;
+      break;
+    }
+    // ---- global relabel ----
+    HIP_TRY(hipEventRecord(e0, st));
+    k_bfs_init<<<vgrid, 256, 0, st>>>(ba, ccnt[0], ccnt[1]);
+    k_bfs_seed<<<kgrid, 256, 0, st>>>(ba, G.src, G.snk, G.k);
+    int level = 0, levels_total = 0;
+    for (;;) {
+      k_bfs_wg<L><<<1, kWgBlock, 0, st>>>(lay, ba, disp, level, fmax);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_SIZE, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      level = hstate[CTL_STATE];
+      int size = hstate[CTL_STATE + 1];
+      if (size == 0) break;
+      for (;;) {  // frontier too large for one workgroup: one launch per level
+        k_bfs_level<L><<<grid, kBlock, 0, st>>>(lay, ba, disp, level);
+        HIP_TRY(hipGetLastError());
+        ++level;
+        HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_SIZE, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        size = 0;
+        for (int c = 0; c < cfg.nclass; ++c) size += hstate[CTL_FCNT + (level % 3) * kNumClasses + c];
+        if (size <= fmax_low) break;
+      }
+      if (size == 0) break;
+    }
+    levels_total = level;
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    relabel_ms += ms;
+    ++rounds;
+    if (hstate[CTL_FLAGS] != 0 || hstate[CTL_FLAGS + 1] != 0) {
+      status = MF_ERR_INTERNAL;  // residual path from a source, or a worklist overflow
+      break;
+    }
+    int64_t active = 0;
+    for (int c = 0; c < cfg.nclass; ++c) active += hstate[CTL_CCNT + c];
+    if (o.verbose)
+      printf("round %lld: relabel levels %d, active vertices %lld\n", (long long)rounds,
+             levels_total, (long long)active);
+    if (active == 0) break;
+    // ---- push-relabel cycles ----
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int q = 0; q < K; ++q) {
+      ca.in = sets[2 + (q & 1)];
+      ca.in.cnt = ccnt[q % 3];
+      ca.out = sets[2 + ((q + 1) & 1)];
+      ca.out.cnt = ccnt[(q + 1) % 3];
+      ca.clear_cnt = ccnt[(q + 2) % 3];
+      if (o.profile_kernels) HIP_TRY(hipEventRecord(ctx->prof_ev[(size_t)2 * q], st));
+      k_cycle<L><<<grid, kBlock, 0, st>>>(lay, ca, disp);
+      if (o.profile_kernels) HIP_TRY(hipEventRecord(ctx->prof_ev[(size_t)2 * q + 1], st));
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    kernel_ms += ms;
+    if (o.profile_kernels) {
+      for (int q = 0; q < K; ++q) {
+        float kms = 0;
+        HIP_TRY(hipEventElapsedTime(&kms, ctx->prof_ev[(size_t)2 * q], ctx->prof_ev[(size_t)2 * q + 1]));
+        prof_ms += kms;
+      }
+      prof_launches += K;
+    }
+    cycles += K;
+  }
+  const int ggrid = std::max(1, std::min((G.k + 255) / 256, 4 * ctx->num_cu));
+  k_gather_values<<<ggrid, 256, 0, st>>>(G.e, G.snk, G.k, G.values);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  const double t1 = now_ms();
+  unsigned long long hs[16];
+  long long v0 = 0;
+  HIP_TRY(hipMemcpy(&v0, G.values, sizeof(long long), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(hs, G.stats, sizeof(hs), hipMemcpyDeviceToHost));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (res) {
+    memset(res, 0, sizeof(*res));
+    res->value = v0;
+    res->source_outflow = (int64_t)hs[ST_SOURCE_OUT];
+    res->solve_ms = t1 - t0;
+    res->kernel_ms = kernel_ms;
+    res->relabel_ms = relabel_ms;
+    res->rounds = rounds;
+    res->cycles = cycles;
+    res->active_total = (int64_t)hs[ST_ACTIVE];
+    res->slots_scanned = (int64_t)hs[ST_SLOTS];
+    res->pushes = (int64_t)hs[ST_PUSHES];
+    res->relabels = (int64_t)hs[ST_RELABELS];
+    res->bfs_slots = (int64_t)hs[ST_BFS_SLOTS];
+    res->bytes_algorithmic = algorithmic_bytes_cycle(G.layout, res->active_total,
+                                                     res->slots_scanned, res->pushes, res->relabels);
+    res->bytes_bfs = algorithmic_bytes_bfs(G.layout, n, rounds, (int64_t)hs[ST_BFS_VERTS], res->bfs_slots);
+    res->profiled_kernel_ms = prof_ms;
+    res->profiled_launches = prof_launches;
+    res->status = status;
+  }
+  G.solved = (status == MF_OK);
+  return status;
+}
+
+mf_status solve_graph(mf_context *ctx, DevGraph &G, mf_algo algo, const mf_options &o,
+                      mf_result *res) {
+  if (G.layout == MF_LAYOUT_BCSR) return run_solve(ctx, G, G.bcsr(), algo, o, res);
+  if (G.layout == MF_LAYOUT_BCSR2) return run_solve(ctx, G, G.bcsr2(), algo, o, res);
+  return run_solve(ctx, G, G.rcsr(), algo, o, res);
+}
+
+static mf_status check_options(const mf_options &o) {
+  if (o.tile_lanes != 0 && o.tile_lanes != 1 && o.tile_lanes != 8 && o.tile_lanes != 16 &&
+      o.tile_lanes != 32 && o.tile_lanes != 64 && o.tile_lanes != kBlock)
+    return MF_ERR_INVALID_ARGUMENT;
+  if (o.cycles_per_relabel < 0 || o.max_rounds < 0 || o.bfs_block_frontier < 0)
+    return MF_ERR_INVALID_ARGUMENT;
+  return MF_OK;
+}
+
+// Packed-instance solve used by the batch entry point.
+mf_status batch_solve_packed(mf_context *ctx, const mf_csr *packed, const int32_t *src,
+                             const int32_t *snk, int32_t k, mf_algo algo, mf_layout layout,
+                             const mf_options &o, int64_t *values, mf_result *res) {
+  mf_status st = check_options(o);
+  if (st != MF_OK) return st;
+  HIP_TRY(hipSetDevice(ctx->device));
+  DevGraph G;
+  st = upload(ctx, packed, src, snk, k, layout, G);
+  if (st == MF_OK) st = solve_graph(ctx, G, algo, o, res);
+  if (st == MF_OK && values) {
+    std::vector<long long> v((size_t)k);
+    if (hipMemcpy(v.data(), G.values, sizeof(long long) * k, hipMemcpyDeviceToHost) != hipSuccess)
+      st = MF_ERR_DEVICE;
+    for (int32_t i = 0; i < k; ++i) values[i] = v[(size_t)i];
+  }
+  free_dev(&G);
+  return st;
+}
+
+}  // namespace mf
+
+struct mf_graph {
+  mf::DevGraph g;
+};
+
+using namespace mf;
+
+extern "C" {
+
+mf_status mf_context_create(int device, mf_context **out) {
+  if (!out) return MF_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return MF_ERR_NO_DEVICE;
+  if (device < 0 || device >= count) return MF_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  mf_context *ctx = new mf_context();
+  ctx->device = device;
+  ctx->num_cu = prop.multiProcessorCount;
+  snprintf(ctx->name, sizeof(ctx->name), "%s", prop.name);
+  snprintf(ctx->arch, sizeof(ctx->arch), "%s", prop.gcnArchName);
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return MF_ERR_DEVICE;
+  }
+  *out = ctx;
+  return MF_OK;
+}
+
+mf_status mf_context_destroy(mf_context *ctx) {
+  if (!ctx) return MF_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return MF_OK;
+}
+
+mf_status mf_device_synchronize(mf_context *ctx) {
+  if (!ctx) return MF_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return MF_OK;
+}
+
+mf_status mf_device_info(mf_context *ctx, int32_t *num_cu, char *name, size_t name_len) {
+  if (!ctx) return MF_ERR_INVALID_ARGUMENT;
+  if (num_cu) *num_cu = ctx->num_cu;
+  if (name && name_len) snprintf(name, name_len, "%s (%s)", ctx->name, ctx->arch);
+  return MF_OK;
+}
+
+mf_status mf_graph_upload(mf_context *ctx, const mf_csr *g, mf_layout layout, mf_graph **out) {
+  if (!ctx || !out) return MF_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (layout != MF_LAYOUT_RCSR && layout != MF_LAYOUT_BCSR && layout != MF_LAYOUT_BCSR2)
+    return MF_ERR_INVALID_ARGUMENT;
+  mf_status st = validate_csr(g);
+  if (st != MF_OK) return st;
+  HIP_TRY(hipSetDevice(ctx->device));
+  mf_graph *mg = new mf_graph();
+  const int32_t s = g->s, t = g->t;
+  st = upload(ctx, g, &s, &t, 1, layout, mg->g);
+  if (st != MF_OK) {
+    free_dev(&mg->g);
+    delete mg;
+    return st;
+  }
+  *out = mg;
+  return MF_OK;
+}
+
+mf_status mf_graph_solve(mf_context *ctx, mf_graph *g, mf_algo algo, const mf_options *opts,
+                         mf_result *res) {
+  if (!ctx || !g || (algo != MF_ALGO_TC && algo != MF_ALGO_VC)) return MF_ERR_INVALID_ARGUMENT;
+  mf_options o;
+  mf_options_default(&o);
+  if (opts) o = *opts;
+  mf_status st = check_options(o);
+  if (st != MF_OK) return st;
+  HIP_TRY(hipSetDevice(ctx->device));
+  return solve_graph(ctx, g->g, algo, o, res);
+}
+
+mf_status mf_graph_read_flow(mf_context *ctx, mf_graph *mg, int32_t *flow_out,
+                             int32_t return_excess) {
+  if (!ctx || !mg || !flow_out) return MF_ERR_INVALID_ARGUMENT;
+  DevGraph &G = mg->g;
+  if (!G.solved) return MF_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (G.layout == MF_LAYOUT_RCSR) {
+    std::vector<int2> fdc((size_t)G.m);
+    HIP_TRY(hipMemcpy(fdc.data(), G.fdc, sizeof(int2) * G.m, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < G.m; ++i) flow_out[i] = G.h_cap[(size_t)i] - fdc[(size_t)i].y;
+  } else if (G.layout == MF_LAYOUT_BCSR) {
+    std::vector<int2> dc((size_t)G.S);
+    HIP_TRY(hipMemcpy(dc.data(), G.dc, sizeof(int2) * G.S, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < G.m; ++i)
+      flow_out[i] = G.h_cap[(size_t)i] - dc[(size_t)G.h_fslot[(size_t)i]].y;
+  } else {
+    std::vector<int2> cr((size_t)G.S);
+    HIP_TRY(hipMemcpy(cr.data(), G.cr, sizeof(int2) * G.S, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < G.m; ++i)
+      flow_out[i] = G.h_cap[(size_t)i] - cr[(size_t)G.h_fslot[(size_t)i]].x;
+  }
+  if (return_excess) {
+    mf_csr c;
+    c.n = G.n;
+    c.m = G.m;
+    c.offsets = G.h_off.data();
+    c.dst = G.h_dst.data();
+    c.cap = G.h_cap.data();
+    c.s = G.h_src[0];
+    c.t = G.h_snk[0];
+    return_excess_to_source(&c, flow_out);
+  }
+  return MF_OK;
+}
+
+mf_status mf_graph_read_cut(mf_context *ctx, mf_graph *mg, uint8_t *sink_side) {
+  if (!ctx || !mg || !sink_side) return MF_ERR_INVALID_ARGUMENT;
+  DevGraph &G = mg->g;
+  if (!G.solved) return MF_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(ctx->device));
+  std::vector<int> h((size_t)G.n);
+  HIP_TRY(hipMemcpy(h.data(), G.h, sizeof(int) * G.n, hipMemcpyDeviceToHost));
+  // After the final global relabel, h < H exactly on the vertices that reach the sink.
+  for (int32_t v = 0; v < G.n; ++v) sink_side[v] = h[(size_t)v] < G.n ? 1 : 0;
+  return MF_OK;
+}
+
+mf_status mf_graph_read_state(mf_context *ctx, mf_graph *mg, int32_t *height, int32_t *excess) {
+  if (!ctx || !mg) return MF_ERR_INVALID_ARGUMENT;
+  DevGraph &G = mg->g;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (height) HIP_TRY(hipMemcpy(height, G.h, sizeof(int) * G.n, hipMemcpyDeviceToHost));
+  if (excess) HIP_TRY(hipMemcpy(excess, G.e, sizeof(int) * G.n, hipMemcpyDeviceToHost));
+  return MF_OK;
+}
+
+mf_status mf_graph_destroy(mf_context *ctx, mf_graph *mg) {
+  if (!ctx || !mg) return MF_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(ctx->device);
+  free_dev(&mg->g);
+  delete mg;
+  return MF_OK;
+}
+
+mf_status mf_solve(mf_context *ctx, const mf_csr *g, mf_algo algo, mf_layout layout,
+                   const mf_options *opts, mf_result *res, int32_t *flow_out) {
+  const double t0 = now_ms();
+  mf_graph *mg = nullptr;
+  mf_status st = mf_graph_upload(ctx, g, layout, &mg);
+  if (st != MF_OK) return st;
+  const double t1 = now_ms();
+  st = mf_graph_solve(ctx, mg, algo, opts, res);
+  if (res) res->upload_ms = t1 - t0;
+  if (st == MF_OK && flow_out) st = mf_graph_read_flow(ctx, mg, flow_out, 1);
+  mf_graph_destroy(ctx, mg);
+  return st;
+}
+
+// Reference-shaped entry point on host RCSR arrays (see mf.h).
+mf_status mf_rcsr_push_relabel(mf_context *ctx, int algo_type, int V, int E, int source, int sink,
+                               int *cpu_height, int *cpu_excess_flow, const int *cpu_offsets,
+                               const int *cpu_destinations, const int *cpu_capacities,
+                               int *cpu_fflows, int *cpu_bflows, const int *cpu_roffsets,
+                               const int *cpu_rdestinations, const int *cpu_flow_idx,
+                               int *excess_total, const mf_options *opts, mf_result *res) {
+  if (!ctx || !cpu_height || !cpu_excess_flow || !cpu_offsets || !cpu_destinations ||
+      !cpu_capacities || !cpu_fflows || !cpu_bflows || !cpu_roffsets || !cpu_rdestinations ||
+      !cpu_flow_idx || !excess_total || (algo_type != 0 && algo_type != 1))
+    return MF_ERR_INVALID_ARGUMENT;
+  mf_csr c;
+  c.n = V;
+  c.m = E;
+  c.offsets = cpu_offsets;
+  c.dst = cpu_destinations;
+  c.cap = cpu_capacities;
+  c.s = source;
+  c.t = sink;
+  mf_status st = validate_csr(&c);
+  if (st != MF_OK) return st;
+  if (cpu_roffsets[0] != 0 || cpu_roffsets[V] != E) return MF_ERR_INVALID_ARGUMENT;
+  for (int v = 0; v < V; ++v)
+    if (cpu_roffsets[v + 1] < cpu_roffsets[v]) return MF_ERR_INVALID_ARGUMENT;
+  for (int q = 0; q < E; ++q)
+    if (cpu_flow_idx[q] < 0 || cpu_flow_idx[q] >= E || cpu_rdestinations[q] < 0 ||
+        cpu_rdestinations[q] >= V)
+      return MF_ERR_INVALID_ARGUMENT;
+  mf_options o;
+  mf_options_default(&o);
+  if (opts) o = *opts;
+  if ((st = check_options(o)) != MF_OK) return st;
+  HIP_TRY(hipSetDevice(ctx->device));
+  DevGraph G;
+  hipStream_t s = ctx->stream;
+  G.layout = MF_LAYOUT_RCSR;
+  G.n = V;
+  G.m = E;
+  G.S = E;
+  G.k = 1;
+  G.h_off.assign(cpu_offsets, cpu_offsets + V + 1);
+  G.h_dst.assign(cpu_destinations, cpu_destinations + E);
+  G.h_cap.assign(cpu_capacities, cpu_capacities + E);
+  G.h_src.assign(1, source);
+  G.h_snk.assign(1, sink);
+  int64_t sc = 0;
+  for (int32_t i = cpu_offsets[source]; i < cpu_offsets[source + 1]; ++i)
+    if (cpu_destinations[i] != source) sc += cpu_capacities[i];
+  auto fail = [&](mf_status code) {
+    free_dev(&G);
+    return code;
+  };
+  if (sc > INT32_MAX) return fail(MF_ERR_OVERFLOW);
+  std::vector<int2> fdc((size_t)E), rdi((size_t)E);
+  for (int i = 0; i < E; ++i) fdc[(size_t)i] = make_int2(cpu_destinations[i], cpu_capacities[i]);
+  for (int q = 0; q < E; ++q) rdi[(size_t)q] = make_int2(cpu_rdestinations[q], cpu_flow_idx[q]);
+  if ((st = dmalloc(&G.off, (size_t)V + 1)) || (st = dmalloc(&G.fdc, (size_t)E)) ||
+      (st = dmalloc(&G.bcf, (size_t)E)) || (st = dmalloc(&G.roff, (size_t)V + 1)) ||
+      (st = dmalloc(&G.rdi, (size_t)E)) || (st = dmalloc(&G.cap0, (size_t)E)) ||
+      (st = dmalloc(&G.h, (size_t)V)) || (st = dmalloc(&G.e, (size_t)V)) ||
+      (st = dmalloc(&G.lists, (size_t)4 * V)) || (st = dmalloc(&G.ctl, (size_t)CTL_SIZE)) ||
+      (st = dmalloc(&G.stats, (size_t)16)) || (st = dmalloc(&G.vcls, (size_t)V)) ||
+      (st = dmalloc(&G.src, (size_t)1)) || (st = dmalloc(&G.snk, (size_t)1)) ||
+      (st = dmalloc(&G.values, (size_t)1)))
+    return fail(st);
+  const ClassConfig vc = class_config(MF_ALGO_VC, 0);
+  std::vector<unsigned char> cls((size_t)V);
+  for (int32_t v = 0; v < V; ++v) {
+    const int64_t deg = (int64_t)(cpu_offsets[v + 1] - cpu_offsets[v]) +
+                        (cpu_roffsets[v + 1] - cpu_roffsets[v]);
+    const int q = class_of_degree(vc, deg);
+    cls[(size_t)v] = (unsigned char)q;
+    G.vc_pop[q]++;
+  }
+  bool ok = hipMemcpyAsync(G.off, cpu_offsets, sizeof(int) * (V + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(G.fdc, fdc.data(), sizeof(int2) * E, hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(G.roff, cpu_roffsets, sizeof(int) * (V + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(G.rdi, rdi.data(), sizeof(int2) * E, hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(G.cap0, cpu_capacities, sizeof(int) * E, hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(G.vcls, cls.data(), (size_t)V, hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(G.src, &source, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(G.snk, &sink, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipStreamSynchronize(s) == hipSuccess;
+  if (!ok) return fail(MF_ERR_DEVICE);
+  mf_result r;
+  st = solve_graph(ctx, G, algo_type == 0 ? MF_ALGO_TC : MF_ALGO_VC, o, &r);
+  if (res) *res = r;
+  if (st == MF_OK) {
+    std::vector<int2> fd((size_t)E);
+    ok = hipMemcpy(fd.data(), G.fdc, sizeof(int2) * E, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(cpu_bflows, G.bcf, sizeof(int) * E, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(cpu_height, G.h, sizeof(int) * V, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(cpu_excess_flow, G.e, sizeof(int) * V, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) return fail(MF_ERR_DEVICE);
+    for (int i = 0; i < E; ++i) cpu_fflows[i] = fd[(size_t)i].y;
+    *excess_total = cpu_excess_flow[sink];
+  }
+  return fail(st);
+}
+
+}  // extern "C"
