@@ -110,6 +110,8 @@ typedef struct mf_result {
   int64_t bytes_bfs;          /* algorithmic bytes of the global relabels */
   double profiled_kernel_ms;  /* with profile_kernels: summed per-launch event time */
   int64_t profiled_launches;  /* with profile_kernels: number of timed launches */
+  int64_t bfs_levels;         /* BFS levels over all global relabels */
+  int64_t bfs_grid_levels;    /* of which run as one grid-wide launch each */
   int32_t status;
   int32_t reserved[7];
 } mf_result;

@@ -7,6 +7,7 @@
  *                          (maxflow-serial/graph.cpp:316-619), quirks included
  *   oracle_dinic_maxflow   independent exact max flow (Dinic's blocking flows)
  *   oracle_certify         capacity / conservation / value / min-cut certificate of a flow
+ *   oracle_rcsr_build, oracle_bcsr_build   the reference's residual-layout builders
  *
  * All arrays are CSR: offsets[n+1], dst[m], cap[m] (int32), 0-based vertex ids.
  */
@@ -38,6 +39,12 @@ long long oracle_dinic_maxflow(int n, int m, const int *offsets, const int *dst,
  * cut_out (optional) receives the capacity of the certificate cut. */
 int oracle_certify(int n, int m, const int *offsets, const int *dst, const int *cap, int s, int t,
                    const int *flow, long long value, int preflow, long long *cut_out);
+
+/* Reference residual-layout builders (see layouts.c). */
+void oracle_rcsr_build(int n, int m, const int *offsets, const int *dst, int *roffsets, int *rdst,
+                       int *flow_index);
+void oracle_bcsr_build(int n, int m, const int *arc_u, const int *arc_v, const int *arc_cap,
+                       int *offsets, int *dst, int *residual, int *backward_idx);
 
 #ifdef __cplusplus
 }

@@ -36,7 +36,37 @@ _lib.oracle_certify.restype = C.c_int
 _lib.oracle_certify.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, C.c_int, C.c_int, _i32p,
                                 C.c_longlong, C.c_int, C.POINTER(C.c_longlong)]
 
+_lib.oracle_rcsr_build.restype = None
+_lib.oracle_rcsr_build.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p, _i32p]
+_lib.oracle_bcsr_build.restype = None
+_lib.oracle_bcsr_build.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p,
+                                   _i32p]
+
 CERT_CODES = {0: "ok", 1: "capacity", 2: "conservation", 3: "value", 4: "cut"}
+
+
+def rcsr_build(g):
+    """Reference RCSR reverse arrays: (roffsets, rdst, flow_index)."""
+    keep, po, pd, pc = _csr(g)
+    roff = np.zeros(g.n + 1, dtype=np.int32)
+    rdst = np.zeros(max(g.m, 1), dtype=np.int32)
+    fidx = np.zeros(max(g.m, 1), dtype=np.int32)
+    _lib.oracle_rcsr_build(g.n, g.m, po, pd, roff.ctypes.data_as(_i32p),
+                           rdst.ctypes.data_as(_i32p), fidx.ctypes.data_as(_i32p))
+    return roff, rdst[: g.m], fidx[: g.m]
+
+
+def bcsr_build(n, arc_u, arc_v, arc_cap):
+    """Reference BCSR from arcs in file order: (offsets, dst, residual, backward_idx)."""
+    u, pu = _arr(arc_u)
+    v, pv = _arr(arc_v)
+    c, pc = _arr(arc_cap)
+    m = len(u)
+    off = np.zeros(n + 1, dtype=np.int32)
+    out = [np.zeros(max(2 * m, 1), dtype=np.int32) for _ in range(3)]
+    _lib.oracle_bcsr_build(n, m, pu, pv, pc, off.ctypes.data_as(_i32p),
+                           *(o.ctypes.data_as(_i32p) for o in out))
+    return (off,) + tuple(o[: 2 * m] for o in out)
 
 
 def _arr(a):

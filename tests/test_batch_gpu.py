@@ -1,0 +1,97 @@
+"""Batched and served solves: every value is the instance's exact max flow, identical alone
+and in the mix."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import generated_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_instances(mf, golden, rng, k):
+    names = ["rlg16_s3", "rlg32_s7", "genrmf8_s7", "genrmf12_s2", "rmat10_s5",
+             "bip_500_700_s5", "bip_3000_2000_s9"]
+    out = []
+    for i in range(k):
+        name = names[int(rng.integers(0, len(names)))]
+        g, v = generated_graph(mf, golden, name)
+        out.append((g, v))
+    return out
+
+
+def test_batch_matches_alone(mf, orc, ctx, golden):
+    rng = np.random.default_rng(3)
+    inst = mixed_instances(mf, golden, rng, 24)
+    graphs = [g for g, _ in inst]
+    values, res = ctx.batch_solve(graphs)
+    assert values.tolist() == [v for _, v in inst]
+    alone = [ctx.solve(g, mf.ALGO_VC).value for g in graphs[:6]]
+    assert alone == values[:6].tolist()
+
+
+def test_batch_bipartite_log_uniform(mf, orc, ctx):
+    rng = np.random.default_rng(12)
+    graphs = []
+    for i in range(64):
+        L, R = (int(round(np.exp(rng.uniform(np.log(64), np.log(2048))))) for _ in range(2))
+        graphs.append(mf.gen_bipartite(L, R, 8.0, seed=1000 + i))
+    expect = [orc.dinic_maxflow(g) for g in graphs]
+    for algo in (0, 1):
+        values, _ = ctx.batch_solve(graphs, batch_mode=algo)
+        assert values.tolist() == expect
+
+
+def test_batch_single_and_errors(mf, ctx, golden):
+    g, v = generated_graph(mf, golden, "genrmf8_s7")
+    values, _ = ctx.batch_solve([g])
+    assert values.tolist() == [v]
+    bad = g.with_terminals(2, 2)
+    with pytest.raises(mf.MFError):
+        ctx.batch_solve([g, bad])
+
+
+def test_server_continuous_batching(mf, ctx, golden):
+    rng = np.random.default_rng(21)
+    inst = mixed_instances(mf, golden, rng, 40)
+    srv = mf.Server(ctx, hbm_budget_bytes=64 << 20, max_inflight_launches=3)
+    ids = []
+    try:
+        for i, (g, v) in enumerate(inst):
+            ids.append((srv.submit(g), v))
+            if i % 5 == 4:
+                srv.progress()  # requests keep arriving while earlier ones run
+        pending = dict(ids)
+        t0 = time.time()
+        while pending and time.time() - t0 < 120:
+            srv.progress(wait=True)
+            for rid in list(pending):
+                r = srv.result(rid)
+                if r is not None:
+                    assert r[0] == pending.pop(rid)
+        assert not pending
+        st = srv.stats()
+        assert st["completed"] == len(inst) and st["submitted"] == len(inst)
+        assert st["arena_bytes"] == 0 and 0 < st["arena_peak_bytes"] <= 64 << 20
+    finally:
+        srv.close()
+
+
+def test_server_budget(mf, ctx, golden):
+    g, v = generated_graph(mf, golden, "genrmf16_s1")
+    small = mf.footprint_bytes(g.n, g.m) - 1
+    srv = mf.Server(ctx, hbm_budget_bytes=small, max_inflight_launches=1)
+    try:
+        with pytest.raises(mf.MFError) as ei:
+            srv.submit(g)
+        assert ei.value.name == "MF_ERR_BUDGET"
+        h, hv = generated_graph(mf, golden, "genrmf8_s7")
+        rid = srv.submit(h)
+        srv.progress(wait=True)
+        while srv.result(rid) is None:
+            srv.progress(wait=True)
+        assert srv.result(rid)[0] == hv
+        assert srv.stats()["rejected"] == 1
+    finally:
+        srv.close()

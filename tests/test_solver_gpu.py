@@ -1,0 +1,195 @@
+"""GPU parity of the push-relabel solver against the CPU oracle (values are exact; flows are
+certified by capacity, conservation and a cut of equal capacity, since flow assignments may
+legally differ)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, fixture_graph, generated_graph
+
+pytestmark = pytest.mark.gpu
+
+LAYOUTS = ["rcsr", "bcsr", "bcsr2"]
+ALGOS = ["tc", "vc"]
+FIXTURES = ["test_DIMACS.txt", "genrmf_a10_b1.max", "genrmf_a4_b3.max", "ww.max", "m10.max",
+            "m30.max", "m50.max", "waissi_example.max", "snap_8.txt", "snap_5.txt", "snap_4.txt"]
+GENERATED = ["rlg16_s3", "rlg32_s7", "rlg64_s1", "genrmf8_s7", "genrmf12_s2", "genrmf16_s1",
+             "rmat10_s5", "rmat12_s3", "bip_500_700_s5", "bip_3000_2000_s9"]
+
+
+def cut_capacity(g, sink_side):
+    tails = g.tails()
+    sel = (sink_side[tails] == 0) & (sink_side[g.dst] == 1)
+    return int(g.cap[sel].astype(np.int64).sum())
+
+
+def solve_and_certify(mf, orc, ctx, g, algo, layout, expect, **opts):
+    dg = ctx.upload(g, mf.LAYOUTS[layout])
+    try:
+        r = dg.solve(mf.ALGOS[algo], **opts)
+        assert r.value == expect, f"{algo}/{layout}: value {r.value} != {expect}"
+        flow = dg.read_flow(return_excess=True)
+        cert, cut = orc.certify(g, flow, r.value)
+        assert cert == "ok", f"{algo}/{layout}: certificate {cert} (cut {cut})"
+        side = dg.read_cut()
+        assert side[g.t] == 1 and side[g.s] == 0
+        assert cut_capacity(g, side) == expect
+        pre = dg.read_flow(return_excess=False)
+        assert orc.certify(g, pre, r.value, preflow=True)[0] == "ok"
+        return r
+    finally:
+        dg.close()
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixtures(mf, orc, ctx, golden, name, algo, layout):
+    g, value = fixture_graph(mf, golden, name)
+    solve_and_certify(mf, orc, ctx, g, algo, layout, value)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("name", GENERATED)
+def test_generated(mf, orc, ctx, golden, name, algo, layout):
+    g, value = generated_graph(mf, golden, name)
+    solve_and_certify(mf, orc, ctx, g, algo, layout, value)
+
+
+def random_graph(mf, rng, n, m, cap_max):
+    u = rng.integers(0, n, size=m)
+    v = rng.integers(0, n, size=m)
+    cap = rng.integers(0, cap_max + 1, size=m)
+    return mf.Graph.from_arcs(n, u, v, cap, 0, n - 1)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_multigraphs(mf, orc, ctx, seed):
+    # parallel arcs, antiparallel arcs, self-loops and zero capacities (the reference's RCSR
+    # aliases parallel arcs; this build pairs every arc with its own reverse slot)
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(10, 400))
+    g = random_graph(mf, rng, n, int(rng.integers(2 * n, 8 * n)), cap_max=int(rng.integers(1, 5000)))
+    expect = orc.dinic_maxflow(g)
+    for algo in ALGOS:
+        for layout in LAYOUTS:
+            solve_and_certify(mf, orc, ctx, g, algo, layout, expect)
+
+
+def test_launch_geometry_stress(mf, orc, ctx, golden):
+    # repeated solves with varied cadence, tile width and BFS hand-over: the lock-free
+    # schedule differs every time, the value may not
+    g, value = generated_graph(mf, golden, "genrmf12_s2")
+    dg = ctx.upload(g, mf.LAYOUT_BCSR)
+    rng = np.random.default_rng(7)
+    try:
+        for _ in range(24):
+            opts = dict(cycles_per_relabel=int(rng.choice([1, 3, 16, 64, 1024])),
+                        tile_lanes=int(rng.choice([0, 1, 8, 16, 32, 64, 256])),
+                        bfs_block_frontier=int(rng.choice([1, 8, 64, 4096])))
+            r = dg.solve(mf.ALGO_VC, **opts)
+            assert r.value == value, opts
+    finally:
+        dg.close()
+
+
+def test_resolve_same_graph(mf, ctx, golden):
+    g, value = generated_graph(mf, golden, "rmat12_s3")
+    dg = ctx.upload(g, mf.LAYOUT_BCSR)
+    try:
+        vals = {dg.solve(mf.ALGO_VC).value for _ in range(5)} | {dg.solve(mf.ALGO_TC).value}
+        assert vals == {value}
+    finally:
+        dg.close()
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+def test_genrmf64_full_size(mf, orc, ctx, golden, algo):
+    g, value = generated_graph(mf, golden, "genrmf64_s1")
+    r = solve_and_certify(mf, orc, ctx, g, algo, "bcsr", value)
+    assert r.rounds >= 1 and r.pushes > 0
+
+
+def test_genrmf64_rcsr(mf, orc, ctx, golden):
+    g, value = generated_graph(mf, golden, "genrmf64_s1")
+    solve_and_certify(mf, orc, ctx, g, "vc", "rcsr", value)
+
+
+def test_rmat22_full_size(mf, orc, ctx, golden):
+    g, value = generated_graph(mf, golden, "rmat22_s1")
+    solve_and_certify(mf, orc, ctx, g, "vc", "bcsr", value)
+
+
+def test_rcsr_reference_entry_point(mf, orc, ctx, golden):
+    # preflow() + push_relabel() of the reference on its own RCSR host arrays
+    g, value = fixture_graph(mf, golden, "m30.max")
+    roff, rdst, fidx = orc.rcsr_build(g)
+    V, E = g.n, g.m
+    h = np.zeros(V, np.int32)
+    e = np.zeros(V, np.int32)
+    ff = g.cap.copy()
+    bf = np.zeros(E, np.int32)
+    for algo in (0, 1):
+        et, r = ctx.rcsr_push_relabel(algo, V, E, g.s, g.t, h, e, g.offsets, g.dst, g.cap, ff, bf,
+                                      roff, rdst, fidx)
+        assert et == value and e[g.t] == value
+        assert np.all(ff >= 0) and np.all(bf >= 0) and np.array_equal(ff + bf, g.cap)
+        flow = bf.copy()
+        assert orc.certify(g, flow, value, preflow=True)[0] == "ok"
+
+
+def test_error_paths(mf, ctx, golden):
+    g, _ = generated_graph(mf, golden, "rlg16_s3")
+    with pytest.raises(mf.MFError) as ei:
+        ctx.upload(g.with_terminals(3, 3))
+    assert ei.value.name == "MF_ERR_INVALID_ARGUMENT"
+    with pytest.raises(mf.MFError) as ei:
+        ctx.upload(g.with_terminals(0, g.n))
+    assert ei.value.name == "MF_ERR_INVALID_ARGUMENT"
+    dg = ctx.upload(g)
+    with pytest.raises(mf.MFError) as ei:
+        dg.solve(mf.ALGO_VC, tile_lanes=12)
+    assert ei.value.name == "MF_ERR_INVALID_ARGUMENT"
+    with pytest.raises(mf.MFError) as ei:
+        dg.solve(mf.ALGO_VC, max_rounds=1, cycles_per_relabel=1)
+    assert ei.value.name == "MF_ERR_ITERATION_LIMIT"
+    dg.close()
+    # excess beyond int32: two source arcs of 2^31 - 1 each
+    big = mf.Graph.from_arcs(3, [0, 0, 1, 2], [1, 2, 2, 1], [2**31 - 1, 2**31 - 1, 5, 5], 0, 2)
+    with pytest.raises(mf.MFError) as ei:
+        ctx.upload(big)
+    assert ei.value.name == "MF_ERR_OVERFLOW"
+
+
+def test_no_path_value_zero(mf, orc, ctx):
+    g = mf.Graph.from_arcs(5, [0, 1, 3], [1, 2, 4], [5, 5, 5], 0, 4)
+    for algo in ALGOS:
+        r = ctx.solve(g, mf.ALGOS[algo])
+        assert r.value == 0
+
+
+@pytest.mark.parametrize("algo", ["0", "1"])
+def test_cli_output_contract(golden, algo):
+    # stdout lines parsed by the reference's bash_run.py:18-22
+    path = os.path.join(GOLDEN, "m50.max")
+    r = subprocess.run([os.path.join(ROOT, "wbpr_amd", "bin", "maxflow"), "-v", "2", "-f", path,
+                        "-a", algo, "-c"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert re.search(r"Total kernel time: \d+\.\d{6} ms", r.stdout)
+    m = re.search(r"The maximum flow value of this flow network as calculated by the parallel "
+                  r"implementation is (\d+), (\d+)", r.stdout)
+    assert m and int(m.group(1)) == golden["fixtures"]["m50.max"]["value"]
+    assert "Certificate: ok" in r.stdout
+
+
+def test_cli_snap_with_terminals(golden):
+    path = os.path.join(GOLDEN, "snap_8.txt")
+    r = subprocess.run([os.path.join(ROOT, "wbpr_amd", "bin", "maxflow"), "-v", "0", "-s", "0",
+                        "-t", "7", "-f", path, "-a", "1", "-l", "0"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "implementation is 4, 4" in r.stdout

@@ -117,6 +117,8 @@ class _Result(C.Structure):
         ("bytes_bfs", C.c_int64),
         ("profiled_kernel_ms", C.c_double),
         ("profiled_launches", C.c_int64),
+        ("bfs_levels", C.c_int64),
+        ("bfs_grid_levels", C.c_int64),
         ("status", C.c_int32),
         ("reserved", C.c_int32 * 7),
     ]
@@ -394,6 +396,8 @@ class Result:
     bytes_bfs: int
     profiled_kernel_ms: float
     profiled_launches: int
+    bfs_levels: int
+    bfs_grid_levels: int
 
     @staticmethod
     def _from(r: _Result) -> "Result":

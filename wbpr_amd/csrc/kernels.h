@@ -42,16 +42,19 @@ struct LayoutBCSR {
     key = j;
   }
   __device__ __forceinline__ int target(int key) const { return dc[key].x; }
-  __device__ __forceinline__ int residual(int key) const { return ld_acq(&dc[key].y); }
+  __device__ __forceinline__ int residual(int key) const { return dc[key].y; }
   __device__ __forceinline__ void push(int key, int d) const {
     atomicSub(&dc[key].y, d);
     atomicAdd(&dc[rev[key]].y, d);
   }
-  // BFS: slot j of frontier vertex u -> neighbour w and residual of the arc w -> u
-  __device__ __forceinline__ void bfs(int, int j, int &w, int &rin) const {
+  // BFS, two stages so a lane can keep several slots in flight:
+  //   bfs_slot(j)   -> neighbour w of slot j and the index of the residual of the arc w -> u
+  //   bfs_rin(idx)  -> that residual
+  __device__ __forceinline__ void bfs_slot(int, int j, int &w, int &idx) const {
     w = dc[j].x;
-    rin = ld_acq(&dc[rev[j]].y);
+    idx = rev[j];
   }
+  __device__ __forceinline__ int bfs_rin(int, int idx) const { return dc[idx].y; }
   __device__ __forceinline__ int degree(int u) const { return off[u + 1] - off[u]; }
 };
 
@@ -68,16 +71,17 @@ struct LayoutBCSR2 {
     key = j;
   }
   __device__ __forceinline__ int target(int key) const { return dst[key]; }
-  __device__ __forceinline__ int residual(int key) const { return ld_acq(&cr[key].x); }
+  __device__ __forceinline__ int residual(int key) const { return cr[key].x; }
   __device__ __forceinline__ void push(int key, int d) const {
     const int rv = cr[key].y;
     atomicSub(&cr[key].x, d);
     atomicAdd(&cr[rv].x, d);
   }
-  __device__ __forceinline__ void bfs(int, int j, int &w, int &rin) const {
+  __device__ __forceinline__ void bfs_slot(int, int j, int &w, int &idx) const {
     w = dst[j];
-    rin = ld_acq(&cr[cr[j].y].x);
+    idx = cr[j].y;
   }
+  __device__ __forceinline__ int bfs_rin(int, int idx) const { return cr[idx].x; }
   __device__ __forceinline__ int degree(int u) const { return off[u + 1] - off[u]; }
 };
 
@@ -112,7 +116,7 @@ struct LayoutRCSR {
     return key < m ? fdc[key].x : rdi[key - m].x;
   }
   __device__ __forceinline__ int residual(int key) const {
-    return key < m ? ld_acq(&fdc[key].y) : ld_acq(&bcf[rdi[key - m].y]);
+    return key < m ? fdc[key].y : bcf[rdi[key - m].y];
   }
   __device__ __forceinline__ void push(int key, int d) const {
     if (key < m) {
@@ -124,15 +128,18 @@ struct LayoutRCSR {
       atomicAdd(&fdc[a].y, d);
     }
   }
-  __device__ __forceinline__ void bfs(int seg, int j, int &w, int &rin) const {
+  __device__ __forceinline__ void bfs_slot(int seg, int j, int &w, int &idx) const {
     if (seg == 0) {  // forward arc u -> w: the arc w -> u is its backward direction
       w = fdc[j].x;
-      rin = ld_acq(&bcf[j]);
+      idx = j;
     } else {  // in-arc a = w -> u: its forward residual
-      int2 y = rdi[j];
+      const int2 y = rdi[j];
       w = y.x;
-      rin = ld_acq(&fdc[y.y].y);
+      idx = y.y;
     }
+  }
+  __device__ __forceinline__ int bfs_rin(int seg, int idx) const {
+    return seg == 0 ? bcf[idx] : fdc[idx].y;
   }
   __device__ __forceinline__ int degree(int u) const {
     return off[u + 1] - off[u] + roff[u + 1] - roff[u];
@@ -152,9 +159,10 @@ struct ListSet {
   int *ovf;
 };
 
-// Number of items of class c, clamped to the region capacity.
+// Number of items of class c, clamped to the region capacity.  The counter was completed by an
+// earlier kernel or, inside the one-workgroup BFS, lives in LDS behind a workgroup barrier.
 __device__ __forceinline__ int list_count(const ListSet &L, int c) {
-  const int k = ld_acq(&L.cnt[c]);
+  const int k = L.cnt[c];
   return k < L.cls_cap[c] ? k : L.cls_cap[c];
 }
 
@@ -169,6 +177,7 @@ struct Dispatch {
 __device__ __forceinline__ void append(const ListSet &L, int item, bool want, int cls,
                                        int nclass) {
   const int lane = __lane_id();
+  if (__ballot(want) == 0) return;
   for (int c = 0; c < nclass; ++c) {
     const bool mine = want && cls == c;
     const unsigned long long mask = __ballot(mine);

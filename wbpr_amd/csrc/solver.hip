@@ -58,48 +58,83 @@ struct CycleArgs {
 // u iff its own atomicSub leaves a positive remainder (or it relabelled below H); a pusher adds
 // w iff its atomicAdd found e(w) == 0.  Whichever of the two atomics on e(u) comes second sees
 // the other's effect, so exactly one of them appends.
+// Per-lane scan state: the lane's lowest (height, key) over residual slots and, for that slot,
+// the neighbour and the residual read during the scan.
+struct Cand {
+  unsigned long long best = ~0ull;
+  int w = 0, res = 0;
+};
+
+template <class L>
+__device__ __forceinline__ void scan_slots(const L &g, const int *h, int u, int first, int step,
+                                           Cand &c, int &deg) {
+  deg = 0;
+#pragma unroll
+  for (int seg = 0; seg < L::kSeg; ++seg) {
+    const int2 r = g.range(u, seg);
+    deg += r.y - r.x;
+    for (int j = r.x + first; j < r.y; j += step) {
+      int w, res, key;
+      g.scan(seg, j, w, res, key);
+      if (res > 0 && w != u) {
+        const unsigned long long k = pack_hk(h[w], key);
+        if (k < c.best) {
+          c.best = k;
+          c.w = w;
+          c.res = res;
+        }
+      }
+    }
+  }
+}
+
+// Push or relabel u given the minimum `best` of the packed (height, key) over its residual
+// arcs.  The lane whose own candidate is the minimum performs the push with the neighbour and
+// residual it already holds; `leader` (one lane of the tile) performs a relabel.  Then u and/or
+// the push target are appended to the next worklist.
+// Exactly-once argument: a vertex is in the current list iff its excess became positive since
+// it was last processed.  The owner re-adds u iff its own atomicSub leaves a positive remainder
+// (or it relabelled below H); a pusher adds w iff its atomicAdd found e(w) == 0.  Whichever of
+// the two atomics on e(u) comes second sees the other's effect, so exactly one of them appends.
 template <class L>
 __device__ __forceinline__ void finish_vertex(const L &g, const CycleArgs &a, int u, int hu,
-                                              unsigned long long best, bool leader, int deg,
-                                              Stats &st) {
+                                              int eu, unsigned long long best, const Cand &mine,
+                                              bool leader, int deg, Stats &st) {
   bool want_u = false, want_w = false;
-  int w = 0;
+  const int hmin = (int)(best >> 32);
   if (leader) {
     st.active++;
     st.slots += deg;
-    if (best == ~0ull) {  // no residual arc left: u can never reach a sink again
+  }
+  if (best == ~0ull || hmin + 1 >= a.H) {
+    // no residual arc to a reached vertex: u can never reach a sink again
+    if (leader) {
       a.h[u] = a.H;
       st.relabels++;
-    } else {
-      const int hmin = (int)(best >> 32);
+    }
+  } else {
+    // Relabel first when no neighbour is lower (h(u) = hmin + 1), then push along the minimum
+    // arc, which is admissible after the relabel: the two lock-free steps relabel and push
+    // with the same reads, done in one cycle.
+    if (hu <= hmin && leader) {
+      a.h[u] = hmin + 1;
+      st.relabels++;
+    }
+    if (mine.best == best) {  // exactly one lane: keys are unique
       const int key = (int)(unsigned)(best & 0xffffffffull);
-      if (hu > hmin) {
-        w = g.target(key);
-        const int eu = ld_acq(&a.e[u]);
-        const int rr = g.residual(key);
-        const int d = min(eu, rr);
-        if (d > 0) {
-          g.push(key, d);
-          const int old_w = atomicAdd(&a.e[w], d);
-          const int rem = atomicSub(&a.e[u], d) - d;
-          want_w = old_w == 0 && hmin > 0;  // height 0 only at sinks
-          want_u = rem > 0;
-          st.pushes++;
-        } else {
-          want_u = eu > 0;
-        }
-      } else {
-        const int nh = min(hmin + 1, a.H);
-        a.h[u] = nh;
-        st.relabels++;
-        want_u = nh < a.H;
-      }
+      const int d = min(eu, mine.res);  // both only grow behind this owner's back
+      const int old_w = atomicAdd(&a.e[mine.w], d);
+      const int old_u = atomicSub(&a.e[u], d);
+      g.push(key, d);
+      want_w = old_w == 0 && hmin > 0;  // height 0 only at sinks
+      want_u = old_u - d > 0;
+      st.pushes++;
     }
   }
   const int cu = a.vcls ? a.vcls[u] : 0;
   append(a.out, u, want_u, cu, a.nclass);
-  const int cw = (want_w && a.vcls) ? a.vcls[w] : 0;
-  append(a.out, w, want_w, cw, a.nclass);
+  const int cw = (want_w && a.vcls) ? a.vcls[mine.w] : 0;
+  append(a.out, mine.w, want_w, cw, a.nclass);
 }
 
 // One discharge step of u by a tile of T lanes (T <= 64, inside one wave).
@@ -107,24 +142,13 @@ template <int T, class L>
 __device__ __forceinline__ void discharge_tile(const L &g, const CycleArgs &a, int u, int lane,
                                                Stats &st) {
   const int hu = a.h[u];
+  const int eu = a.e[u];
   if (hu >= a.H) return;  // became inactive after it was queued (all lanes read the same value)
-  unsigned long long best = ~0ull;
-  int deg = 0;
-#pragma unroll
-  for (int seg = 0; seg < L::kSeg; ++seg) {
-    const int2 r = g.range(u, seg);
-    deg += r.y - r.x;
-    for (int j = r.x + lane; j < r.y; j += T) {
-      int w, res, key;
-      g.scan(seg, j, w, res, key);
-      if (res > 0 && w != u) {
-        const unsigned long long k = pack_hk(a.h[w], key);
-        best = k < best ? k : best;
-      }
-    }
-  }
-  best = tile_min<T>(best);
-  finish_vertex(g, a, u, hu, best, lane == 0, deg, st);
+  Cand c;
+  int deg;
+  scan_slots(g, a.h, u, lane, T, c, deg);
+  const unsigned long long best = tile_min<T>(c.best);
+  finish_vertex(g, a, u, hu, eu, best, c, lane == 0, deg, st);
 }
 
 // One discharge step of a hub vertex by the whole block (kBlock lanes, LDS reduction).
@@ -132,30 +156,17 @@ template <class L>
 __device__ __forceinline__ void discharge_block(const L &g, const CycleArgs &a, int u, Stats &st) {
   __shared__ unsigned long long red[kBlock / 64];
   const int hu = a.h[u];
+  const int eu = a.e[u];
   if (hu >= a.H) return;  // uniform over the block: only this block writes h[u]
-  unsigned long long best = ~0ull;
-  int deg = 0;
-#pragma unroll
-  for (int seg = 0; seg < L::kSeg; ++seg) {
-    const int2 r = g.range(u, seg);
-    deg += r.y - r.x;
-    for (int j = r.x + (int)threadIdx.x; j < r.y; j += kBlock) {
-      int w, res, key;
-      g.scan(seg, j, w, res, key);
-      if (res > 0 && w != u) {
-        const unsigned long long k = pack_hk(a.h[w], key);
-        best = k < best ? k : best;
-      }
-    }
-  }
-  best = tile_min<64>(best);
+  Cand c;
+  int deg;
+  scan_slots(g, a.h, u, (int)threadIdx.x, kBlock, c, deg);
+  unsigned long long best = tile_min<64>(c.best);
   if (__lane_id() == 0) red[threadIdx.x / 64] = best;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < kBlock / 64; ++k) best = red[k] < best ? red[k] : best;
-  }
+  for (int k = 0; k < kBlock / 64; ++k) best = red[k] < best ? red[k] : best;
   __syncthreads();
-  finish_vertex(g, a, u, hu, best, threadIdx.x == 0, deg, st);
+  finish_vertex(g, a, u, hu, eu, best, c, threadIdx.x == 0, deg, st);
 }
 
 template <int T, class L>
@@ -217,29 +228,44 @@ struct BfsArgs {
   unsigned long long *stats;
 };
 
+// Slots one BFS lane keeps in flight: the four dependent steps (slot record, reverse residual,
+// neighbour height, claim) are each issued for the whole batch before waiting on the next.
+constexpr int kBfsBatch = 4;
+
 template <int T, class L>
 __device__ __forceinline__ void bfs_vertex(const L &g, const BfsArgs &a, const ListSet &out, int u,
                                            int lane, int level, long long &ns) {
+  constexpr int B = kBfsBatch;
 #pragma unroll
   for (int seg = 0; seg < L::kSeg; ++seg) {
     const int2 r = g.range(u, seg);
     if (lane == 0) ns += r.y - r.x;
-    for (int j = r.x + lane; j < r.y; j += T) {
-      int w, rin;
-      g.bfs(seg, j, w, rin);
-      bool claim = false, act = false;
-      if (rin > 0 && w != u) {
-        const int hw = ld_acq(&a.h[w]);
-        if (hw == a.H + 1) {
-          a.flags[0] = 1;  // residual path from a source: preflow invariant broken
-        } else if (hw == a.H && atomicCAS(&a.h[w], a.H, level + 1) == a.H) {
-          claim = true;
-          act = ld_acq(&a.e[w]) > 0;
-        }
+    for (int base = r.x + lane; base < r.y; base += T * B) {
+      int w[B], idx[B], rin[B], hw[B];
+      bool claim[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        w[b] = -1;
+        idx[b] = 0;
+        if (base + b * T < r.y) g.bfs_slot(seg, base + b * T, w[b], idx[b]);
       }
-      const int cw = (claim && a.vcls) ? a.vcls[w] : 0;
-      append(out, w, claim, cw, a.nclass);
-      append(a.active, w, act, cw, a.nclass);
+#pragma unroll
+      for (int b = 0; b < B; ++b) rin[b] = (w[b] >= 0 && w[b] != u) ? g.bfs_rin(seg, idx[b]) : 0;
+#pragma unroll
+      for (int b = 0; b < B; ++b) hw[b] = rin[b] > 0 ? a.h[w[b]] : -1;
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        claim[b] = false;
+        if (hw[b] == a.H + 1) a.flags[0] = 1;  // residual path from a source: invariant broken
+        else if (hw[b] == a.H) claim[b] = atomicCAS(&a.h[w[b]], a.H, level + 1) == a.H;
+      }
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        const bool act = claim[b] && a.e[w[b]] > 0;
+        const int cw = (claim[b] && a.vcls) ? a.vcls[w[b]] : 0;
+        append(out, w[b], claim[b], cw, a.nclass);
+        append(a.active, w[b], act, cw, a.nclass);
+      }
     }
   }
 }
@@ -259,19 +285,21 @@ template <class L>
 __device__ __forceinline__ void bfs_dispatch(const L &g, const BfsArgs &a, const ListSet &in,
                                              const ListSet &out, int c, int lanes, int cnt, int tid,
                                              int nthreads, int level, long long &nv, long long &ns) {
+  // A BFS lane covers kBfsBatch slots per step, so a class processed with `lanes` lanes by the
+  // cycle kernels uses lanes / kBfsBatch lanes per vertex here (same slots per step, four times
+  // as many vertices in flight).
   switch (lanes) {
-#define MF_BFS_CASE(TT)                                                                       \
-  case TT:                                                                                    \
+#define MF_BFS_CASE(LANES, TT)                                                                \
+  case LANES:                                                                                 \
     bfs_class<TT>(g, a, in, out, c, cnt, tid / TT, nthreads / TT, tid % TT, level, nv, ns); \
     break;
-    MF_BFS_CASE(1)
-    MF_BFS_CASE(8)
-    MF_BFS_CASE(16)
-    MF_BFS_CASE(32)
-    MF_BFS_CASE(64)
+    MF_BFS_CASE(1, 1)
+    MF_BFS_CASE(8, 2)
+    MF_BFS_CASE(16, 4)
+    MF_BFS_CASE(32, 8)
+    MF_BFS_CASE(64, 16)
     default:
-      bfs_class<kBlock>(g, a, in, out, c, cnt, tid / kBlock, nthreads / kBlock, tid % kBlock,
-                        level, nv, ns);
+      bfs_class<64>(g, a, in, out, c, cnt, tid / 64, nthreads / 64, tid % 64, level, nv, ns);
       break;
 #undef MF_BFS_CASE
   }
@@ -325,37 +353,48 @@ __global__ __launch_bounds__(kBlock) void k_bfs_level(L g, BfsArgs a, Dispatch d
 }
 
 // All BFS levels inside one workgroup while the frontier holds at most fmax vertices: workgroup
-// barriers replace kernel launches, which is what long-diameter graphs need.
+// barriers replace kernel launches, which is what long-diameter graphs need.  The frontier
+// counters live in LDS for the duration (the frontier items stay in global memory, which the
+// workgroup's waves share through their common CU); on exit the current counters are handed
+// back to the grid-wide kernels.  No device-scope fence is needed: every value this kernel
+// reads that another kernel wrote was written before the launch.
 template <class L>
 __global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d, int level,
                                                      int fmax) {
+  __shared__ int s_cnt[3][kNumClasses];
   __shared__ int s_size;
   long long nv = 0, ns = 0;
   const int tid = (int)threadIdx.x;
+  if (tid < 3 * kNumClasses) {
+    const int q = tid / kNumClasses, c = tid % kNumClasses;
+    s_cnt[q][c] = (q == level % 3) ? a.fcnt[q][c] : 0;
+  }
+  __syncthreads();
   while (true) {
     if (tid == 0) {
-      int *clr = a.fcnt[(level + 2) % 3];
-      for (int c = 0; c < kNumClasses; ++c) atomicExch(&clr[c], 0);
       int sz = 0;
-      for (int c = 0; c < d.nclass; ++c) sz += ld_acq(&a.fcnt[level % 3][c]);
+      for (int c = 0; c < d.nclass; ++c) sz += s_cnt[level % 3][c];
       s_size = sz;
-      __threadfence();
     }
+    if (tid < kNumClasses) s_cnt[(level + 2) % 3][tid] = 0;  // next level's output counters
     __syncthreads();
     const int size = s_size;
     if (size == 0 || size > fmax) break;
     ListSet in = a.fr[level & 1];
-    in.cnt = a.fcnt[level % 3];
+    in.cnt = s_cnt[level % 3];
     ListSet out = a.fr[(level + 1) & 1];
-    out.cnt = a.fcnt[(level + 1) % 3];
+    out.cnt = s_cnt[(level + 1) % 3];
     for (int c = 0; c < d.nclass; ++c) {
       const int cnt = list_count(in, c);
       if (cnt == 0) continue;
       bfs_dispatch(g, a, in, out, c, d.lanes[c], cnt, tid, kWgBlock, level, nv, ns);
     }
-    __threadfence();
     __syncthreads();
     ++level;
+  }
+  if (tid < 3 * kNumClasses) {
+    const int q = tid / kNumClasses, c = tid % kNumClasses;
+    a.fcnt[q][c] = (q == level % 3) ? s_cnt[q][c] : 0;
   }
   if (tid == 0) {
     a.state[0] = level;
@@ -440,6 +479,12 @@ static double now_ms() {
   using namespace std::chrono;
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
+
+// Defaults measured on Genrmf a=b=64 (MI355X): one global relabel per 1024 cycles, and the
+// one-workgroup BFS while the frontier holds at most 4096 vertices.
+constexpr int kDefaultCycles = 1024;
+constexpr int kDefaultBfsFrontier = 4096;
+constexpr int kCycleChunk = 64;  // cycles launched between two worklist checks
 
 template <class T>
 static mf_status dmalloc(T **p, size_t count) {
@@ -732,9 +777,9 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   ca.nclass = cfg.nclass;
   ca.stats = G.stats;
 
-  const int fmax = o.bfs_block_frontier > 0 ? o.bfs_block_frontier : 512;
+  const int fmax = o.bfs_block_frontier > 0 ? o.bfs_block_frontier : kDefaultBfsFrontier;
   const int fmax_low = std::max(1, fmax / 2);
-  const int K = o.cycles_per_relabel > 0 ? o.cycles_per_relabel : 64;
+  const int K = o.cycles_per_relabel > 0 ? o.cycles_per_relabel : kDefaultCycles;
   const int64_t max_rounds = o.max_rounds > 0 ? o.max_rounds : (int64_t)100000000;
   const int vgrid = std::max(1, std::min((n + 255) / 256, 16 * ctx->num_cu));
   const int kgrid = std::max(1, std::min((G.k + 255) / 256, 16 * ctx->num_cu));
@@ -743,11 +788,11 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   double kernel_ms = 0, relabel_ms = 0, prof_ms = 0;
-  int64_t prof_launches = 0, cycles = 0, rounds = 0;
+  int64_t prof_launches = 0, cycles = 0, rounds = 0, bfs_levels = 0, bfs_grid_levels = 0;
   mf_status status = MF_OK;
-  if (o.profile_kernels && ctx->prof_ev.size() < (size_t)2 * K) {
+  if (o.profile_kernels && ctx->prof_ev.size() < (size_t)2 * kCycleChunk) {
     const size_t old = ctx->prof_ev.size();
-    ctx->prof_ev.resize((size_t)2 * K);
+    ctx->prof_ev.resize((size_t)2 * kCycleChunk);
     for (size_t q = old; q < ctx->prof_ev.size(); ++q) HIP_TRY(hipEventCreate(&ctx->prof_ev[q]));
   }
 
@@ -786,6 +831,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
         k_bfs_level<L><<<grid, kBlock, 0, st>>>(lay, ba, disp, level);
         HIP_TRY(hipGetLastError());
         ++level;
+        ++bfs_grid_levels;
         HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_SIZE, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         size = 0;
@@ -795,6 +841,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       if (size == 0) break;
     }
     levels_total = level;
+    bfs_levels += level;
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipEventSynchronize(e1));
     float ms = 0;
@@ -811,32 +858,44 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       printf("round %lld: relabel levels %d, active vertices %lld\n", (long long)rounds,
              levels_total, (long long)active);
     if (active == 0) break;
-    // ---- push-relabel cycles ----
+    // ---- push-relabel cycles, issued in chunks; the round ends early once the worklist is
+    //      empty (checked between chunks with one small read-back) ----
     HIP_TRY(hipEventRecord(e0, st));
-    for (int q = 0; q < K; ++q) {
-      ca.in = sets[2 + (q & 1)];
-      ca.in.cnt = ccnt[q % 3];
-      ca.out = sets[2 + ((q + 1) & 1)];
-      ca.out.cnt = ccnt[(q + 1) % 3];
-      ca.clear_cnt = ccnt[(q + 2) % 3];
-      if (o.profile_kernels) HIP_TRY(hipEventRecord(ctx->prof_ev[(size_t)2 * q], st));
-      k_cycle<L><<<grid, kBlock, 0, st>>>(lay, ca, disp);
-      if (o.profile_kernels) HIP_TRY(hipEventRecord(ctx->prof_ev[(size_t)2 * q + 1], st));
+    int done = 0;
+    while (done < K) {
+      const int chunk = std::min(kCycleChunk, K - done);
+      for (int q = done; q < done + chunk; ++q) {
+        ca.in = sets[2 + (q & 1)];
+        ca.in.cnt = ccnt[q % 3];
+        ca.out = sets[2 + ((q + 1) & 1)];
+        ca.out.cnt = ccnt[(q + 1) % 3];
+        ca.clear_cnt = ccnt[(q + 2) % 3];
+        const size_t pe = (size_t)2 * (q - done);
+        if (o.profile_kernels) HIP_TRY(hipEventRecord(ctx->prof_ev[pe], st));
+        k_cycle<L><<<grid, kBlock, 0, st>>>(lay, ca, disp);
+        if (o.profile_kernels) HIP_TRY(hipEventRecord(ctx->prof_ev[pe + 1], st));
+      }
+      HIP_TRY(hipGetLastError());
+      done += chunk;
+      HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_SIZE, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      if (o.profile_kernels) {
+        for (int q = 0; q < chunk; ++q) {
+          float kms = 0;
+          HIP_TRY(hipEventElapsedTime(&kms, ctx->prof_ev[(size_t)2 * q], ctx->prof_ev[(size_t)2 * q + 1]));
+          prof_ms += kms;
+        }
+        prof_launches += chunk;
+      }
+      int64_t left = 0;
+      for (int c = 0; c < cfg.nclass; ++c) left += hstate[CTL_CCNT + (done % 3) * kNumClasses + c];
+      if (left == 0) break;
     }
-    HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipEventSynchronize(e1));
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     kernel_ms += ms;
-    if (o.profile_kernels) {
-      for (int q = 0; q < K; ++q) {
-        float kms = 0;
-        HIP_TRY(hipEventElapsedTime(&kms, ctx->prof_ev[(size_t)2 * q], ctx->prof_ev[(size_t)2 * q + 1]));
-        prof_ms += kms;
-      }
-      prof_launches += K;
-    }
-    cycles += K;
+    cycles += done;
   }
   const int ggrid = std::max(1, std::min((G.k + 255) / 256, 4 * ctx->num_cu));
   k_gather_values<<<ggrid, 256, 0, st>>>(G.e, G.snk, G.k, G.values);
@@ -868,6 +927,8 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     res->bytes_bfs = algorithmic_bytes_bfs(G.layout, n, rounds, (int64_t)hs[ST_BFS_VERTS], res->bfs_slots);
     res->profiled_kernel_ms = prof_ms;
     res->profiled_launches = prof_launches;
+    res->bfs_levels = bfs_levels;
+    res->bfs_grid_levels = bfs_grid_levels;
     res->status = status;
   }
   G.solved = (status == MF_OK);
