@@ -83,13 +83,16 @@ typedef struct mf_host_graph {
 } mf_host_graph;
 
 typedef struct mf_options {
-  int32_t cycles_per_relabel; /* push-relabel cycles between global relabels; 0 = automatic */
-  int32_t tile_lanes;         /* VC lanes per vertex: 0 = by degree bucket, else 1/8/16/32/64 */
+  int32_t cycles_per_relabel; /* fixed push-relabel cycles between global relabels; 0 = adaptive */
+  int32_t tile_lanes;         /* VC lanes per vertex: 0 = by degree bucket, else 1/8/16/32/64/256 */
   int64_t max_rounds;         /* global-relabel rounds before MF_ERR_ITERATION_LIMIT; 0 = default */
   int32_t verbose;            /* 1: print one line per round like push_relabel.cu:121,194 */
   int32_t profile_kernels;    /* 1: time every push-relabel launch with HIP events (adds overhead) */
   int32_t bfs_block_frontier; /* frontier size up to which the one-workgroup BFS runs; 0 = default */
-  int32_t reserved[8];
+  float relabel_time_ratio;   /* adaptive cadence: after a global relabel taking T ms, cycles run
+                                 for about ratio * T ms; 0 = default (2) */
+  int32_t batch_algo;         /* mf_batch_solve: 0 = VC, 1 = TC */
+  int32_t reserved[6];
 } mf_options;
 
 typedef struct mf_result {
@@ -183,6 +186,10 @@ mf_status mf_graph_solve(mf_context *ctx, mf_graph *g, mf_algo algo, const mf_op
  * preflow; with return_excess != 0 the excess stranded at vertices that cannot reach t is sent
  * back to s on the host so the result satisfies conservation everywhere except s and t. */
 mf_status mf_graph_read_flow(mf_context *ctx, mf_graph *g, int32_t *flow_out, int32_t return_excess);
+/* Host utility: turns a preflow on g (flow[m], excess >= 0 everywhere but s) into a flow of the
+ * same value by cancelling flow cycles and returning every stranded excess to s along
+ * flow-carrying arcs.  flow is modified in place.  Used by mf_graph_read_flow. */
+mf_status mf_flow_return_excess(const mf_csr *g, int32_t *flow);
 /* Min-cut side after a solve: side[v] = 1 if v can reach t in the residual graph, else 0. */
 mf_status mf_graph_read_cut(mf_context *ctx, mf_graph *g, uint8_t *sink_side);
 /* Heights and excesses after a solve (n entries each; either may be NULL). */

@@ -39,7 +39,7 @@ def test_batch_bipartite_log_uniform(mf, orc, ctx):
         graphs.append(mf.gen_bipartite(L, R, 8.0, seed=1000 + i))
     expect = [orc.dinic_maxflow(g) for g in graphs]
     for algo in (0, 1):
-        values, _ = ctx.batch_solve(graphs, batch_mode=algo)
+        values, _ = ctx.batch_solve(graphs, batch_algo=algo)
         assert values.tolist() == expect
 
 

@@ -31,14 +31,15 @@ def solve_and_certify(mf, orc, ctx, g, algo, layout, expect, **opts):
     try:
         r = dg.solve(mf.ALGOS[algo], **opts)
         assert r.value == expect, f"{algo}/{layout}: value {r.value} != {expect}"
+        pre = dg.read_flow(return_excess=False)
+        cert, cut = orc.certify(g, pre, r.value, preflow=True)
+        assert cert == "ok", f"{algo}/{layout}: preflow certificate {cert} (cut {cut})"
         flow = dg.read_flow(return_excess=True)
         cert, cut = orc.certify(g, flow, r.value)
-        assert cert == "ok", f"{algo}/{layout}: certificate {cert} (cut {cut})"
+        assert cert == "ok", f"{algo}/{layout}: flow certificate {cert} (cut {cut})"
         side = dg.read_cut()
         assert side[g.t] == 1 and side[g.s] == 0
         assert cut_capacity(g, side) == expect
-        pre = dg.read_flow(return_excess=False)
-        assert orc.certify(g, pre, r.value, preflow=True)[0] == "ok"
         return r
     finally:
         dg.close()
@@ -117,6 +118,15 @@ def test_genrmf64_full_size(mf, orc, ctx, golden, algo):
 def test_genrmf64_rcsr(mf, orc, ctx, golden):
     g, value = generated_graph(mf, golden, "genrmf64_s1")
     solve_and_certify(mf, orc, ctx, g, "vc", "rcsr", value)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_rmat18_hub_vertices(mf, orc, ctx, layout):
+    # hubs above 1024 slots are discharged by a whole block; every wave of the block must split
+    # the same excess (regression: per-wave reads of a growing e(u) broke conservation)
+    g = mf.gen_rmat(18, 16, seed=1)
+    assert np.diff(g.offsets).max() > 1024
+    solve_and_certify(mf, orc, ctx, g, "vc", layout, orc.dinic_maxflow(g))
 
 
 def test_rmat22_full_size(mf, orc, ctx, golden):

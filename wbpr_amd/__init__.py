@@ -94,7 +94,9 @@ class _Options(C.Structure):
         ("verbose", C.c_int32),
         ("profile_kernels", C.c_int32),
         ("bfs_block_frontier", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("relabel_time_ratio", C.c_float),
+        ("batch_algo", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -199,6 +201,7 @@ _sig("mf_server_result", C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), C.POINTE
 _sig("mf_server_stats_get", C.c_int, _vp, C.POINTER(_ServerStats))
 _sig("mf_server_destroy", C.c_int, _vp)
 _sig("mf_footprint_bytes", C.c_int64, C.c_int32, C.c_int64)
+_sig("mf_flow_return_excess", C.c_int, C.POINTER(_Csr), _i32p)
 
 #: every function the C header declares (checked by the export test)
 EXPORTED = [
@@ -210,7 +213,7 @@ EXPORTED = [
     "mf_graph_solve", "mf_graph_read_flow", "mf_graph_read_cut", "mf_graph_read_state",
     "mf_graph_destroy", "mf_solve", "mf_rcsr_push_relabel", "mf_batch_solve",
     "mf_server_create", "mf_server_submit", "mf_server_progress", "mf_server_result",
-    "mf_server_stats_get", "mf_server_destroy", "mf_footprint_bytes",
+    "mf_server_stats_get", "mf_server_destroy", "mf_footprint_bytes", "mf_flow_return_excess",
 ]
 
 
@@ -372,6 +375,14 @@ def build_layout(g: Graph, layout: int) -> dict:
     return out
 
 
+def return_excess(g: Graph, flow: np.ndarray) -> np.ndarray:
+    """Preflow -> flow of the same value (host repair used by DeviceGraph.read_flow)."""
+    fl = np.ascontiguousarray(flow, dtype=np.int32).copy()
+    c = g.csr()
+    _check(_lib.mf_flow_return_excess(C.byref(c), _ptr(fl)), "flow_return_excess")
+    return fl
+
+
 def footprint_bytes(n: int, m: int) -> int:
     return int(_lib.mf_footprint_bytes(n, m))
 
@@ -408,10 +419,8 @@ def options(**kw) -> _Options:
     o = _Options()
     _lib.mf_options_default(C.byref(o))
     for k, v in kw.items():
-        if k == "batch_large_vertices":
-            o.reserved[0] = int(v)
-        elif k == "batch_mode":
-            o.reserved[1] = int(v)
+        if k == "relabel_time_ratio":
+            o.relabel_time_ratio = float(v)
         else:
             setattr(o, k, int(v))
     return o

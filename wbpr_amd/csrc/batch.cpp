@@ -64,7 +64,7 @@ extern "C" mf_status mf_batch_solve(mf_context *ctx, const mf_csr *inst, int32_t
   mf_csr packed;
   mf_status st = pack_instances(inst, k, off, dst, cap, src, snk, packed);
   if (st != MF_OK) return st;
-  const mf_algo algo = o.reserved[1] == 1 ? MF_ALGO_TC : MF_ALGO_VC;
+  const mf_algo algo = o.batch_algo == 1 ? MF_ALGO_TC : MF_ALGO_VC;
   return batch_solve_packed(ctx, &packed, src.data(), snk.data(), k, algo, MF_LAYOUT_BCSR, o,
                             values, total);
 }

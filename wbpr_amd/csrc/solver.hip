@@ -52,19 +52,23 @@ struct CycleArgs {
   unsigned long long *stats;
 };
 
-// Push or relabel u given the packed (height, key) minimum over its residual arcs, then append
-// u and/or the push target to the next worklist.  Exactly-once argument: a vertex is in the
-// current list iff its excess became positive since it was last processed.  The owner re-adds
-// u iff its own atomicSub leaves a positive remainder (or it relabelled below H); a pusher adds
-// w iff its atomicAdd found e(w) == 0.  Whichever of the two atomics on e(u) comes second sees
-// the other's effect, so exactly one of them appends.
-// Per-lane scan state: the lane's lowest (height, key) over residual slots and, for that slot,
-// the neighbour and the residual read during the scan.
+// Per-lane scan state: the lane's lowest (height, key) over its residual slots and, for that
+// slot, the neighbour and the residual read during the scan.
 struct Cand {
   unsigned long long best = ~0ull;
   int w = 0, res = 0;
+  __device__ __forceinline__ void consider(int hw, int key, int w_, int res_) {
+    const unsigned long long k = pack_hk(hw, key);
+    if (k < best) {
+      best = k;
+      w = w_;
+      res = res_;
+    }
+  }
 };
 
+// Lane `first` of a group of `step` lanes scans slots first, first + step, ... of u, two slots
+// in flight (both slot records, then both neighbour heights) so the dependent loads overlap.
 template <class L>
 __device__ __forceinline__ void scan_slots(const L &g, const int *h, int u, int first, int step,
                                            Cand &c, int &deg) {
@@ -73,62 +77,83 @@ __device__ __forceinline__ void scan_slots(const L &g, const int *h, int u, int 
   for (int seg = 0; seg < L::kSeg; ++seg) {
     const int2 r = g.range(u, seg);
     deg += r.y - r.x;
-    for (int j = r.x + first; j < r.y; j += step) {
-      int w, res, key;
-      g.scan(seg, j, w, res, key);
-      if (res > 0 && w != u) {
-        const unsigned long long k = pack_hk(h[w], key);
-        if (k < c.best) {
-          c.best = k;
-          c.w = w;
-          c.res = res;
-        }
-      }
+    int j = r.x + first;
+    for (; j + step < r.y; j += 2 * step) {
+      int w0, r0, k0, w1, r1, k1;
+      g.scan(seg, j, w0, r0, k0);
+      g.scan(seg, j + step, w1, r1, k1);
+      const bool v0 = r0 > 0 && w0 != u, v1 = r1 > 0 && w1 != u;
+      const int h0 = v0 ? h[w0] : 0;
+      const int h1 = v1 ? h[w1] : 0;
+      if (v0) c.consider(h0, k0, w0, r0);
+      if (v1) c.consider(h1, k1, w1, r1);
+    }
+    if (j < r.y) {
+      int w0, r0, k0;
+      g.scan(seg, j, w0, r0, k0);
+      if (r0 > 0 && w0 != u) c.consider(h[w0], k0, w0, r0);
     }
   }
 }
 
-// Push or relabel u given the minimum `best` of the packed (height, key) over its residual
-// arcs.  The lane whose own candidate is the minimum performs the push with the neighbour and
-// residual it already holds; `leader` (one lane of the tile) performs a relabel.  Then u and/or
-// the push target are appended to the next worklist.
-// Exactly-once argument: a vertex is in the current list iff its excess became positive since
-// it was last processed.  The owner re-adds u iff its own atomicSub leaves a positive remainder
-// (or it relabelled below H); a pusher adds w iff its atomicAdd found e(w) == 0.  Whichever of
-// the two atomics on e(u) comes second sees the other's effect, so exactly one of them appends.
+template <int T>
+__device__ __forceinline__ long long tile_scan(long long v) {
+  const int lane = __lane_id() % T;
+#pragma unroll
+  for (int off = 1; off < T; off <<= 1) {
+    const long long y = __shfl_up(v, off, T);
+    if (lane >= off) v += y;
+  }
+  return v;
+}
+
+// What the group decided for u after the min-reduction.
+struct Plan {
+  bool dead;   // no residual arc to a reached vertex: u can never reach a sink again
+  int hnew;    // height after this step (h(u), or hmin + 1 when no neighbour is lower)
+  int total;   // excess pushed out of u by all lanes together
+  int d;       // this lane's share
+};
+
+// One discharge step, finished: relabel (if no neighbour is lower), then every lane whose own
+// candidate arc is admissible at the new height pushes its share of the excess (the shares are
+// a prefix split of e(u) over the lanes' residuals, so the lane holding the minimum arc is
+// always among them).  Relabelling and pushing in one step is the lock-free relabel followed by
+// the push it enables, with the same reads.  Then u and the push targets are appended to the
+// next worklist.  Exactly-once: a vertex is in the current list iff its excess became positive
+// since it was last processed.  The owner re-adds u iff its own atomicSub leaves a positive
+// remainder (or it relabelled below H); a pusher adds w iff its atomicAdd found e(w) == 0.
+// Whichever of two atomics on one excess comes second sees the other's effect, so exactly one
+// of them appends.
 template <class L>
 __device__ __forceinline__ void finish_vertex(const L &g, const CycleArgs &a, int u, int hu,
-                                              int eu, unsigned long long best, const Cand &mine,
-                                              bool leader, int deg, Stats &st) {
+                                              const Plan &p, const Cand &mine, bool leader,
+                                              int deg, Stats &st) {
   bool want_u = false, want_w = false;
-  const int hmin = (int)(best >> 32);
   if (leader) {
     st.active++;
     st.slots += deg;
   }
-  if (best == ~0ull || hmin + 1 >= a.H) {
-    // no residual arc to a reached vertex: u can never reach a sink again
+  if (p.dead) {
     if (leader) {
       a.h[u] = a.H;
       st.relabels++;
     }
   } else {
-    // Relabel first when no neighbour is lower (h(u) = hmin + 1), then push along the minimum
-    // arc, which is admissible after the relabel: the two lock-free steps relabel and push
-    // with the same reads, done in one cycle.
-    if (hu <= hmin && leader) {
-      a.h[u] = hmin + 1;
+    if (leader && p.hnew > hu) {
+      a.h[u] = p.hnew;
       st.relabels++;
     }
-    if (mine.best == best) {  // exactly one lane: keys are unique
-      const int key = (int)(unsigned)(best & 0xffffffffull);
-      const int d = min(eu, mine.res);  // both only grow behind this owner's back
-      const int old_w = atomicAdd(&a.e[mine.w], d);
-      const int old_u = atomicSub(&a.e[u], d);
-      g.push(key, d);
-      want_w = old_w == 0 && hmin > 0;  // height 0 only at sinks
-      want_u = old_u - d > 0;
+    if (p.d > 0) {
+      const int key = (int)(unsigned)(mine.best & 0xffffffffull);
+      const int old_w = atomicAdd(&a.e[mine.w], p.d);
+      g.push(key, p.d);
+      want_w = old_w == 0 && (int)(mine.best >> 32) > 0;  // height 0 only at sinks
       st.pushes++;
+    }
+    if (leader) {
+      const int old_u = atomicSub(&a.e[u], p.total);
+      want_u = old_u - p.total > 0;
     }
   }
   const int cu = a.vcls ? a.vcls[u] : 0;
@@ -137,36 +162,75 @@ __device__ __forceinline__ void finish_vertex(const L &g, const CycleArgs &a, in
   append(a.out, mine.w, want_w, cw, a.nclass);
 }
 
+__device__ __forceinline__ void plan_heights(unsigned long long best, int hu, int H, Plan &p) {
+  const int hmin = (int)(best >> 32);
+  p.dead = best == ~0ull || hmin + 1 >= H;
+  p.hnew = p.dead ? H : max(hu, hmin + 1);
+}
+
+__device__ __forceinline__ long long admissible_residual(const Plan &p, const Cand &c) {
+  return (!p.dead && c.best != ~0ull && (int)(c.best >> 32) < p.hnew) ? c.res : 0;
+}
+
 // One discharge step of u by a tile of T lanes (T <= 64, inside one wave).
 template <int T, class L>
 __device__ __forceinline__ void discharge_tile(const L &g, const CycleArgs &a, int u, int lane,
                                                Stats &st) {
   const int hu = a.h[u];
-  const int eu = a.e[u];
+  // e(u) grows while the tile runs: every lane uses lane 0's read (the value subtracted below)
+  const int eu = __shfl(a.e[u], 0, T);
   if (hu >= a.H) return;  // became inactive after it was queued (all lanes read the same value)
   Cand c;
   int deg;
   scan_slots(g, a.h, u, lane, T, c, deg);
-  const unsigned long long best = tile_min<T>(c.best);
-  finish_vertex(g, a, u, hu, eu, best, c, lane == 0, deg, st);
+  Plan p;
+  plan_heights(tile_min<T>(c.best), hu, a.H, p);
+  const long long want = admissible_residual(p, c);
+  const long long incl = tile_scan<T>(want);
+  const long long sum = __shfl(incl, T - 1, T);
+  p.d = (int)max(0LL, min(want, (long long)eu - (incl - want)));
+  p.total = (int)min((long long)eu, sum);
+  finish_vertex(g, a, u, hu, p, c, lane == 0, deg, st);
 }
 
-// One discharge step of a hub vertex by the whole block (kBlock lanes, LDS reduction).
+// One discharge step of a hub vertex by the whole block (kBlock lanes, LDS reductions).
 template <class L>
 __device__ __forceinline__ void discharge_block(const L &g, const CycleArgs &a, int u, Stats &st) {
   __shared__ unsigned long long red[kBlock / 64];
+  __shared__ long long wsum[kBlock / 64];
+  __shared__ int s_eu;
+  // e(u) grows while the block runs (other vertices push into u): one thread reads it and the
+  // block shares that value, so every wave splits the same excess as the one subtracted
+  if (threadIdx.x == 0) s_eu = a.e[u];
+  __syncthreads();
+  const int eu = s_eu;
   const int hu = a.h[u];
-  const int eu = a.e[u];
+  __syncthreads();
   if (hu >= a.H) return;  // uniform over the block: only this block writes h[u]
   Cand c;
   int deg;
   scan_slots(g, a.h, u, (int)threadIdx.x, kBlock, c, deg);
+  const int wave = threadIdx.x / 64;
   unsigned long long best = tile_min<64>(c.best);
-  if (__lane_id() == 0) red[threadIdx.x / 64] = best;
+  if (__lane_id() == 0) red[wave] = best;
   __syncthreads();
   for (int k = 0; k < kBlock / 64; ++k) best = red[k] < best ? red[k] : best;
+  Plan p;
+  plan_heights(best, hu, a.H, p);
+  const long long want = admissible_residual(p, c);
+  long long incl = tile_scan<64>(want);
+  if (__lane_id() == 63) wsum[wave] = incl;
   __syncthreads();
-  finish_vertex(g, a, u, hu, eu, best, c, threadIdx.x == 0, deg, st);
+  long long before = 0, sum = 0;
+  for (int k = 0; k < kBlock / 64; ++k) {
+    if (k < wave) before += wsum[k];
+    sum += wsum[k];
+  }
+  __syncthreads();
+  incl += before;
+  p.d = (int)max(0LL, min(want, (long long)eu - (incl - want)));
+  p.total = (int)min((long long)eu, sum);
+  finish_vertex(g, a, u, hu, p, c, threadIdx.x == 0, deg, st);
 }
 
 template <int T, class L>
@@ -218,8 +282,10 @@ struct BfsArgs {
   int *h;
   const int *e;
   int H;
-  const unsigned char *vcls;
+  const unsigned char *vcls;  // degree class of the frontier lists (always the VC classes)
   int nclass;
+  const unsigned char *acls;  // degree class of the cycle worklists (null: one class)
+  int nclass_act;
   ListSet fr[2];   // frontier items, double-buffered by level parity
   int *fcnt[3];    // frontier counters: level L reads fcnt[L%3], writes fcnt[(L+1)%3]
   ListSet active;  // reached vertices with excess (first worklist of the cycles)
@@ -263,8 +329,9 @@ __device__ __forceinline__ void bfs_vertex(const L &g, const BfsArgs &a, const L
       for (int b = 0; b < B; ++b) {
         const bool act = claim[b] && a.e[w[b]] > 0;
         const int cw = (claim[b] && a.vcls) ? a.vcls[w[b]] : 0;
+        const int aw = (act && a.acls) ? a.acls[w[b]] : 0;
         append(out, w[b], claim[b], cw, a.nclass);
-        append(a.active, w[b], act, cw, a.nclass);
+        append(a.active, w[b], act, aw, a.nclass_act);
       }
     }
   }
@@ -480,11 +547,15 @@ static double now_ms() {
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-// Defaults measured on Genrmf a=b=64 (MI355X): one global relabel per 1024 cycles, and the
-// one-workgroup BFS while the frontier holds at most 4096 vertices.
-constexpr int kDefaultCycles = 1024;
-constexpr int kDefaultBfsFrontier = 4096;
-constexpr int kCycleChunk = 64;  // cycles launched between two worklist checks
+// Global-relabel cadence.  Adaptive by default: after a global relabel that took T ms, cycles
+// run until about ratio * T ms have passed (checked between chunks of cycles), so the two phases
+// stay balanced whether relabels are cheap (small graphs) or dominated by long BFS chains.
+// Excess that can no longer reach a sink only stops moving at the next global relabel, which is
+// why a fixed large cycle count is wasteful on graphs that strand excess (bipartite, RLG).
+constexpr float kDefaultRelabelRatio = 2.0f;
+constexpr int kMaxAdaptiveCycles = 1 << 14;  // cap of cycles per round in adaptive mode
+constexpr int kDefaultBfsFrontier = 4096;    // one-workgroup BFS up to this frontier size
+constexpr int kCycleChunk = 64;              // cycles launched between two worklist checks
 
 template <class T>
 static mf_status dmalloc(T **p, size_t count) {
@@ -734,16 +805,22 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   const unsigned char *vcls = cfg.nclass == 1 ? nullptr : G.vcls;
   const Dispatch disp = make_dispatch(cfg, pop, ctx->num_cu);
   const int grid = disp.blk_begin[cfg.nclass];
+  // the global relabel always splits its frontier by degree class, whatever the cycle kernels do
+  const ClassConfig bcfg = class_config(MF_ALGO_VC, 0);
+  const Dispatch bdisp = make_dispatch(bcfg, G.vc_pop, ctx->num_cu);
+  const int bgrid = bdisp.blk_begin[bcfg.nclass];
 
   // list sets: F0, F1 (BFS frontier), S0, S1 (cycle worklists); class regions by population
   ListSet sets[4];
   for (int q = 0; q < 4; ++q) {
     sets[q].items = G.lists + (size_t)q * n;
+    const int nc = q < 2 ? bcfg.nclass : cfg.nclass;
+    const int64_t *pp = q < 2 ? G.vc_pop : pop;
     int64_t acc = 0;
     for (int c = 0; c < kNumClasses; ++c) {
       sets[q].cls_off[c] = (int)acc;
-      sets[q].cls_cap[c] = c < cfg.nclass ? (int)pop[c] : 0;
-      if (c < cfg.nclass) acc += pop[c];
+      sets[q].cls_cap[c] = c < nc ? (int)pp[c] : 0;
+      if (c < nc) acc += pp[c];
     }
     sets[q].cnt = nullptr;
     sets[q].ovf = G.ctl + CTL_FLAGS + 1;
@@ -758,8 +835,10 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   ba.h = G.h;
   ba.e = G.e;
   ba.H = n;
-  ba.vcls = vcls;
-  ba.nclass = cfg.nclass;
+  ba.vcls = G.vcls;
+  ba.nclass = bcfg.nclass;
+  ba.acls = vcls;
+  ba.nclass_act = cfg.nclass;
   ba.fr[0] = sets[0];
   ba.fr[1] = sets[1];
   for (int q = 0; q < 3; ++q) ba.fcnt[q] = fcnt[q];
@@ -779,7 +858,10 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
 
   const int fmax = o.bfs_block_frontier > 0 ? o.bfs_block_frontier : kDefaultBfsFrontier;
   const int fmax_low = std::max(1, fmax / 2);
-  const int K = o.cycles_per_relabel > 0 ? o.cycles_per_relabel : kDefaultCycles;
+  const bool adaptive = o.cycles_per_relabel == 0;
+  const int K = adaptive ? kMaxAdaptiveCycles : o.cycles_per_relabel;
+  const double ratio = o.relabel_time_ratio > 0 ? o.relabel_time_ratio : kDefaultRelabelRatio;
+  double last_relabel_ms = 0;
   const int64_t max_rounds = o.max_rounds > 0 ? o.max_rounds : (int64_t)100000000;
   const int vgrid = std::max(1, std::min((n + 255) / 256, 16 * ctx->num_cu));
   const int kgrid = std::max(1, std::min((G.k + 255) / 256, 16 * ctx->num_cu));
@@ -820,7 +902,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     k_bfs_seed<<<kgrid, 256, 0, st>>>(ba, G.src, G.snk, G.k);
     int level = 0, levels_total = 0;
     for (;;) {
-      k_bfs_wg<L><<<1, kWgBlock, 0, st>>>(lay, ba, disp, level, fmax);
+      k_bfs_wg<L><<<1, kWgBlock, 0, st>>>(lay, ba, bdisp, level, fmax);
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_SIZE, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
@@ -828,14 +910,14 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       int size = hstate[CTL_STATE + 1];
       if (size == 0) break;
       for (;;) {  // frontier too large for one workgroup: one launch per level
-        k_bfs_level<L><<<grid, kBlock, 0, st>>>(lay, ba, disp, level);
+        k_bfs_level<L><<<bgrid, kBlock, 0, st>>>(lay, ba, bdisp, level);
         HIP_TRY(hipGetLastError());
         ++level;
         ++bfs_grid_levels;
         HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_SIZE, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         size = 0;
-        for (int c = 0; c < cfg.nclass; ++c) size += hstate[CTL_FCNT + (level % 3) * kNumClasses + c];
+        for (int c = 0; c < bcfg.nclass; ++c) size += hstate[CTL_FCNT + (level % 3) * kNumClasses + c];
         if (size <= fmax_low) break;
       }
       if (size == 0) break;
@@ -847,6 +929,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     relabel_ms += ms;
+    last_relabel_ms = ms;
     ++rounds;
     if (hstate[CTL_FLAGS] != 0 || hstate[CTL_FLAGS + 1] != 0) {
       status = MF_ERR_INTERNAL;  // residual path from a source, or a worklist overflow
@@ -861,6 +944,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     // ---- push-relabel cycles, issued in chunks; the round ends early once the worklist is
     //      empty (checked between chunks with one small read-back) ----
     HIP_TRY(hipEventRecord(e0, st));
+    const double tc0 = now_ms();
     int done = 0;
     while (done < K) {
       const int chunk = std::min(kCycleChunk, K - done);
@@ -890,6 +974,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       int64_t left = 0;
       for (int c = 0; c < cfg.nclass; ++c) left += hstate[CTL_CCNT + (done % 3) * kNumClasses + c];
       if (left == 0) break;
+      if (adaptive && now_ms() - tc0 >= ratio * last_relabel_ms) break;
     }
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipEventSynchronize(e1));
@@ -959,8 +1044,11 @@ mf_status batch_solve_packed(mf_context *ctx, const mf_csr *packed, const int32_
   if (st != MF_OK) return st;
   HIP_TRY(hipSetDevice(ctx->device));
   DevGraph G;
+  const double t0 = now_ms();
   st = upload(ctx, packed, src, snk, k, layout, G);
+  const double t1 = now_ms();
   if (st == MF_OK) st = solve_graph(ctx, G, algo, o, res);
+  if (res) res->upload_ms = t1 - t0;
   if (st == MF_OK && values) {
     std::vector<long long> v((size_t)k);
     if (hipMemcpy(v.data(), G.values, sizeof(long long) * k, hipMemcpyDeviceToHost) != hipSuccess)
