@@ -223,7 +223,7 @@ def bench_batched(args, mf):
         f"({time.perf_counter() - t0:.1f} s)")
     ctx = mf.Context(0)
     dev_name, ncu = ctx.device_info()
-    algo_mode = 1 if args.algo == "tc" else 0
+    algo_mode = 1 if args.algo == "vc" else 0  # batch_algo: 0 TC, 1 VC
     runs = []
     values = None
     for step in range(args.warmup + args.steps):
@@ -245,7 +245,7 @@ def bench_batched(args, mf):
     per = []
     for i in sample:
         dg = ctx.upload(graphs[i], mf.LAYOUT_BCSR)
-        r = dg.solve(mf.ALGO_TC if algo_mode else mf.ALGO_VC)
+        r = dg.solve(mf.ALGO_VC if algo_mode else mf.ALGO_TC)
         if r.value != values[i]:
             raise SystemExit(f"instance {i}: alone {r.value} != batched {values[i]}")
         per.append(r.solve_ms)
@@ -256,7 +256,7 @@ def bench_batched(args, mf):
     cpu = None if args.no_cpu_baseline else cpu_baseline_batched(graphs, values, args.cpu_sample)
     config = {"workload": f"{k} bipartite matching networks, |L|,|R| log-uniform [256,65536], "
                           f"Poisson(8) arcs per left vertex, unit capacities, seed {args.seed}",
-              "vertices": nv, "arcs": na, "algo": "tc" if algo_mode else "vc", "layout": "bcsr",
+              "vertices": nv, "arcs": na, "algo": "vc" if algo_mode else "tc", "layout": "bcsr",
               "batched_solves_per_s": round(batched_sps, 2),
               "one_by_one_solves_per_s": round(one_sps, 2),
               "one_by_one_sample": len(per), "speedup_vs_one_by_one": round(batched_sps / one_sps, 2),

@@ -91,8 +91,10 @@ typedef struct mf_options {
   int32_t bfs_block_frontier; /* frontier size up to which the one-workgroup BFS runs; 0 = default */
   float relabel_time_ratio;   /* adaptive cadence: after a global relabel taking T ms, cycles run
                                  for about ratio * T ms; 0 = default (2) */
-  int32_t batch_algo;         /* mf_batch_solve: 0 = VC, 1 = TC */
-  int32_t reserved[6];
+  int32_t batch_algo;         /* mf_batch_solve: 0 = TC (default), 1 = VC */
+  int32_t batch_mode;         /* mf_batch_solve: 0 = one workgroup per instance (default),
+                                 1 = all instances as one packed graph on the whole grid */
+  int32_t reserved[5];
 } mf_options;
 
 typedef struct mf_result {
@@ -215,9 +217,10 @@ mf_status mf_rcsr_push_relabel(mf_context *ctx, int algo_type, int V, int E, int
 
 /* ---- batched solve (no reference counterpart) ----
  * k independent instances packed into one concatenated residual graph with per-instance
- * offsets; solved by a persistent kernel in which each workgroup pulls whole instances from a
- * device queue.  Instances larger than opts->reserved[0] vertices (0 = automatic) are solved
- * by the grid-wide single-instance path inside the same call.  values[k] receives each value. */
+ * offsets.  opts->batch_mode 0 (default): a persistent kernel whose workgroups pull whole
+ * instances from a device queue, largest first, and solve each inside the workgroup;
+ * batch_mode 1: the grid-wide kernels run over all instances at once.  opts->batch_algo picks
+ * TC (0, default) or VC (1) cycles.  values[k] receives each value. */
 mf_status mf_batch_solve(mf_context *ctx, const mf_csr *inst, int32_t k, const mf_options *opts,
                          int64_t *values, mf_result *total);
 

@@ -38,9 +38,29 @@ def test_batch_bipartite_log_uniform(mf, orc, ctx):
         L, R = (int(round(np.exp(rng.uniform(np.log(64), np.log(2048))))) for _ in range(2))
         graphs.append(mf.gen_bipartite(L, R, 8.0, seed=1000 + i))
     expect = [orc.dinic_maxflow(g) for g in graphs]
+    for mode in (0, 1):  # one workgroup per instance / packed graph on the whole grid
+        for algo in (0, 1):  # TC / VC
+            values, _ = ctx.batch_solve(graphs, batch_algo=algo, batch_mode=mode)
+            assert values.tolist() == expect, (mode, algo)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_mixed_generators_modes(mf, ctx, golden, mode):
+    rng = np.random.default_rng(5 + mode)
+    inst = mixed_instances(mf, golden, rng, 40)
     for algo in (0, 1):
-        values, _ = ctx.batch_solve(graphs, batch_algo=algo)
-        assert values.tolist() == expect
+        values, _ = ctx.batch_solve([g for g, _ in inst], batch_algo=algo, batch_mode=mode,
+                                    tile_lanes=0)
+        assert values.tolist() == [v for _, v in inst], (mode, algo)
+
+
+def test_batch_fixed_cadence_and_limits(mf, ctx, golden):
+    g, v = generated_graph(mf, golden, "genrmf12_s2")
+    values, _ = ctx.batch_solve([g, g, g], cycles_per_relabel=8)
+    assert values.tolist() == [v] * 3
+    with pytest.raises(mf.MFError) as ei:
+        ctx.batch_solve([g], max_rounds=1, cycles_per_relabel=1)
+    assert ei.value.name == "MF_ERR_ITERATION_LIMIT"
 
 
 def test_batch_single_and_errors(mf, ctx, golden):

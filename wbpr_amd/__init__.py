@@ -96,7 +96,8 @@ class _Options(C.Structure):
         ("bfs_block_frontier", C.c_int32),
         ("relabel_time_ratio", C.c_float),
         ("batch_algo", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("batch_mode", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 

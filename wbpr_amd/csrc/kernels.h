@@ -16,6 +16,7 @@ namespace mfk {
 constexpr int kNumClasses = 5;   // degree classes of the worklists
 constexpr int kBlock = 256;      // threads per block of the cycle / BFS-level kernels
 constexpr int kWgBlock = 1024;   // threads of the one-workgroup BFS kernel
+constexpr int kMultiPushLanes = 64;  // tiles this wide (and whole blocks) push along many arcs
 
 __device__ __forceinline__ int ld_acq(const int *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -234,6 +235,7 @@ enum StatIdx {
 __device__ __forceinline__ void block_flush(long long a, long long b, long long c, long long d,
                                             unsigned long long *out, int i0) {
   __shared__ long long red[4][kWgBlock / 64];
+  __syncthreads();  // a previous flush in the same kernel may still be reading red
   a = tile_sum<64>(a);
   b = tile_sum<64>(b);
   c = tile_sum<64>(c);

@@ -17,7 +17,8 @@
 namespace mf {
 mf_status pack_instances(const mf_csr *inst, int32_t k, std::vector<int32_t> &off,
                          std::vector<int32_t> &dst, std::vector<int32_t> &cap,
-                         std::vector<int32_t> &src, std::vector<int32_t> &snk, mf_csr &packed);
+                         std::vector<int32_t> &src, std::vector<int32_t> &snk,
+                         std::vector<int32_t> &vbase, mf_csr &packed);
 }
 
 using namespace mf;
@@ -83,16 +84,18 @@ static void run_launch(mf_server *srv, Launch *L) {
   std::vector<mf_csr> views;
   views.reserve(L->reqs.size());
   for (auto &r : L->reqs) views.push_back(r->view());
-  std::vector<int32_t> off, dst, cap, src, snk;
+  std::vector<int32_t> off, dst, cap, src, snk, vbase;
   mf_csr packed;
   std::vector<int64_t> values(L->reqs.size());
   mf_result res{};
-  mf_status stt = pack_instances(views.data(), (int32_t)views.size(), off, dst, cap, src, snk, packed);
+  mf_status stt =
+      pack_instances(views.data(), (int32_t)views.size(), off, dst, cap, src, snk, vbase, packed);
   if (stt == MF_OK) {
     mf_options o;
     mf_options_default(&o);
     stt = batch_solve_packed(srv->ctxs[(size_t)L->ctx_slot], &packed, src.data(), snk.data(),
-                             (int32_t)views.size(), MF_ALGO_VC, MF_LAYOUT_BCSR, o, values.data(), &res);
+                             vbase.data(), (int32_t)views.size(), MF_ALGO_TC, MF_LAYOUT_BCSR, o,
+                             values.data(), &res);
   }
   std::lock_guard<std::mutex> lk(srv->mu);
   for (size_t i = 0; i < L->reqs.size(); ++i) {

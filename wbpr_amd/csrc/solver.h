@@ -30,7 +30,9 @@ const char *last_device_error();
 struct mf_context;
 namespace mf {
 // Solves k instances packed into one CSR (vertex ranges disjoint, src/snk global ids).
+// vbase[k+1]: first vertex of every instance.
 mf_status batch_solve_packed(mf_context *ctx, const mf_csr *packed, const int32_t *src,
-                             const int32_t *snk, int32_t k, mf_algo algo, mf_layout layout,
-                             const mf_options &o, int64_t *values, mf_result *res);
+                             const int32_t *snk, const int32_t *vbase, int32_t k, mf_algo algo,
+                             mf_layout layout, const mf_options &o, int64_t *values,
+                             mf_result *res);
 }  // namespace mf

@@ -184,12 +184,22 @@ __device__ __forceinline__ void discharge_tile(const L &g, const CycleArgs &a, i
   int deg;
   scan_slots(g, a.h, u, lane, T, c, deg);
   Plan p;
-  plan_heights(tile_min<T>(c.best), hu, a.H, p);
-  const long long want = admissible_residual(p, c);
-  const long long incl = tile_scan<T>(want);
-  const long long sum = __shfl(incl, T - 1, T);
-  p.d = (int)max(0LL, min(want, (long long)eu - (incl - want)));
-  p.total = (int)min((long long)eu, sum);
+  const unsigned long long best = tile_min<T>(c.best);
+  plan_heights(best, hu, a.H, p);
+  if constexpr (T >= kMultiPushLanes) {
+    const long long want = admissible_residual(p, c);
+    const long long incl = tile_scan<T>(want);
+    const long long sum = __shfl(incl, T - 1, T);
+    p.d = (int)max(0LL, min(want, (long long)eu - (incl - want)));
+    p.total = (int)min((long long)eu, sum);
+  } else {
+    // low-degree vertices push along the minimum arc only: fanning small excesses out to many
+    // neighbours (unit-capacity bipartite networks) multiplies the pushes that follow
+    const bool winner = !p.dead && c.best == best;  // one lane: keys are unique
+    const int total = (int)tile_sum<T>(winner ? (long long)min(eu, c.res) : 0LL);
+    p.d = winner ? total : 0;
+    p.total = total;
+  }
   finish_vertex(g, a, u, hu, p, c, lane == 0, deg, st);
 }
 
@@ -521,6 +531,227 @@ __global__ void k_gather_values(const int *e, const int *snk, int k, long long *
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride) out[i] = e[snk[i]];
 }
 
+// ---- batched solve: whole instances per workgroup --------------------------------------------
+//
+// A persistent grid of workgroups pulls instances from a device queue (largest first) and solves
+// each one entirely inside the workgroup: preflow, global relabels (BFS level by level) and
+// push-relabel cycles, with workgroup barriers in place of kernel launches and all worklist
+// counters in LDS.  Only this workgroup touches the instance's vertices and slots, so the
+// barriers give the same ordering the single-instance path gets from kernel boundaries.  The
+// relabel cadence is the adaptive one of the single-instance path, timed with wall_clock64.
+
+struct BatchArgs {
+  int *h, *e;
+  int *lists;                      // 4 sets x N items (instance regions at their vertex ranges)
+  int N;
+  const int *vbase, *nver, *src, *snk;
+  const int *pop;                  // k x kNumClasses, cycle classes
+  const int *bpop;                 // k x kNumClasses, BFS (VC) classes
+  const unsigned char *vcls;       // VC class of every vertex
+  const int *order;                // instances, largest first
+  int k;
+  int *next;                       // queue head
+  long long *values;
+  int *status;                     // per instance: 0 ok, 1 round limit, 2 invariant broken
+  int *ovf;
+  unsigned long long *stats;
+  int nclass;                      // cycle classes (1 for TC)
+  int lanes[kNumClasses];          // cycle lanes per class
+  int blanes[kNumClasses];         // BFS (VC) lanes per class
+  int use_vcls;                    // cycle classes follow vcls
+  float ratio;                     // cycle time / relabel time per round
+  int max_rounds;
+  int max_cycles;                  // per round
+};
+
+template <class L>
+__device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &ca, int c, int lanes,
+                                                  int cnt, Stats &st) {
+  const int base = ca.in.cls_off[c];
+  const int tid = (int)threadIdx.x;
+  switch (lanes) {
+#define MF_BATCH_CASE(TT)                                                          \
+  case TT:                                                                         \
+    for (int i = tid / TT; i < cnt; i += kBlock / TT)                               \
+      discharge_tile<TT>(g, ca, ca.in.items[base + i], tid % TT, st);              \
+    break;
+    MF_BATCH_CASE(1)
+    MF_BATCH_CASE(8)
+    MF_BATCH_CASE(16)
+    MF_BATCH_CASE(32)
+    MF_BATCH_CASE(64)
+#undef MF_BATCH_CASE
+    default:
+      for (int i = 0; i < cnt; ++i) discharge_block(g, ca, ca.in.items[base + i], st);
+      break;
+  }
+}
+
+template <class L>
+__device__ void batch_instance(const L &g, const BatchArgs &b, int inst,
+                               int (*s_f)[kNumClasses], int (*s_c)[kNumClasses], int *s_flag,
+                               int *s_dec, Stats &st, long long &nv, long long &ns) {
+  const int tid = (int)threadIdx.x;
+  const int vb = b.vbase[inst], n = b.nver[inst], s = b.src[inst], t = b.snk[inst];
+  const int H = n;  // heights are distances inside the instance
+  ListSet sets[4];
+  for (int q = 0; q < 4; ++q) {
+    const int *pp = (q < 2 ? b.bpop : b.pop) + (size_t)inst * kNumClasses;
+    const int nc = q < 2 ? kNumClasses : b.nclass;
+    sets[q].items = b.lists + (size_t)q * b.N;
+    int acc = vb;
+    for (int c = 0; c < kNumClasses; ++c) {
+      sets[q].cls_off[c] = acc;
+      sets[q].cls_cap[c] = c < nc ? pp[c] : 0;
+      if (c < nc) acc += pp[c];
+    }
+    sets[q].cnt = nullptr;
+    sets[q].ovf = b.ovf;
+  }
+  BfsArgs ba;
+  ba.h = b.h;
+  ba.e = b.e;
+  ba.H = H;
+  ba.vcls = b.vcls;
+  ba.nclass = kNumClasses;
+  ba.acls = b.use_vcls ? b.vcls : nullptr;
+  ba.nclass_act = b.nclass;
+  ba.fr[0] = sets[0];
+  ba.fr[1] = sets[1];
+  for (int q = 0; q < 3; ++q) ba.fcnt[q] = s_f[q];
+  ba.active = sets[2];
+  ba.active.cnt = s_c[0];
+  ba.flags = s_flag;
+  ba.state = nullptr;
+  ba.stats = nullptr;
+  CycleArgs ca;
+  ca.h = b.h;
+  ca.e = b.e;
+  ca.H = H;
+  ca.vcls = b.use_vcls ? b.vcls : nullptr;
+  ca.nclass = b.nclass;
+  ca.clear_cnt = nullptr;
+  ca.stats = nullptr;
+
+  // preflow: saturate the source's residual arcs
+#pragma unroll
+  for (int seg = 0; seg < L::kSeg; ++seg) {
+    const int2 r = g.range(s, seg);
+    for (int j = r.x + tid; j < r.y; j += kBlock) {
+      int w, res, key;
+      g.scan(seg, j, w, res, key);
+      if (res > 0 && w != s) {
+        g.push(key, res);
+        atomicAdd(&b.e[w], res);
+      }
+    }
+  }
+  __syncthreads();
+  int status = 0;
+  for (int round = 0;; ++round) {
+    if (round >= b.max_rounds) {
+      status = 1;
+      break;
+    }
+    // ---- global relabel inside the workgroup ----
+    const long long t_bfs = wall_clock64();
+    for (int v = vb + tid; v < vb + n; v += kBlock) b.h[v] = H;
+    if (tid < 3 * kNumClasses) {
+      s_f[tid / kNumClasses][tid % kNumClasses] = 0;
+      s_c[tid / kNumClasses][tid % kNumClasses] = 0;
+    }
+    if (tid == 0) s_flag[0] = 0;
+    __syncthreads();
+    if (tid == 0) {
+      b.h[t] = 0;
+      b.h[s] = H + 1;
+      ListSet f0 = sets[0];
+      f0.cnt = s_f[0];
+      append(f0, t, true, b.vcls[t], kNumClasses);
+    }
+    __syncthreads();
+    for (int level = 0;; ++level) {
+      if (tid == 0) {
+        int sz = 0;
+        for (int c = 0; c < kNumClasses; ++c) sz += s_f[level % 3][c];
+        *s_dec = sz;
+      }
+      if (tid < kNumClasses) s_f[(level + 2) % 3][tid] = 0;
+      __syncthreads();
+      const int size = *s_dec;
+      __syncthreads();
+      if (size == 0) break;
+      ListSet in = sets[level & 1];
+      in.cnt = s_f[level % 3];
+      ListSet out = sets[(level + 1) & 1];
+      out.cnt = s_f[(level + 1) % 3];
+      for (int c = 0; c < kNumClasses; ++c) {
+        const int cnt = list_count(in, c);
+        if (cnt) bfs_dispatch(g, ba, in, out, c, b.blanes[c], cnt, tid, kBlock, level, nv, ns);
+      }
+      __syncthreads();
+    }
+    if (s_flag[0]) {
+      status = 2;
+      break;
+    }
+    int active = 0;
+    for (int c = 0; c < b.nclass; ++c) active += s_c[0][c];
+    if (active == 0) break;
+    const long long bfs_ticks = wall_clock64() - t_bfs;
+    // ---- push-relabel cycles ----
+    const long long t_cyc = wall_clock64();
+    for (int q = 0; q < b.max_cycles; ++q) {
+      ca.in = sets[2 + (q & 1)];
+      ca.in.cnt = s_c[q % 3];
+      ca.out = sets[2 + ((q + 1) & 1)];
+      ca.out.cnt = s_c[(q + 1) % 3];
+      if (tid < kNumClasses) s_c[(q + 2) % 3][tid] = 0;  // last read in cycle q - 1
+      for (int c = 0; c < b.nclass; ++c) {
+        const int cnt = list_count(ca.in, c);
+        if (cnt) batch_cycle_class(g, ca, c, b.lanes[c], cnt, st);
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int left = 0;
+        for (int c = 0; c < b.nclass; ++c) left += s_c[(q + 1) % 3][c];
+        const long long spent = wall_clock64() - t_cyc;
+        *s_dec = (left == 0 || (double)spent >= b.ratio * (double)bfs_ticks) ? 1 : 0;
+      }
+      __syncthreads();
+      const int stop = *s_dec;
+      __syncthreads();
+      if (stop) break;
+    }
+  }
+  if (tid == 0) {
+    b.values[inst] = b.e[t];
+    b.status[inst] = status;
+  }
+  __syncthreads();
+}
+
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_batch_wg(L g, BatchArgs b) {
+  __shared__ int s_f[3][kNumClasses];
+  __shared__ int s_c[3][kNumClasses];
+  __shared__ int s_flag[2];
+  __shared__ int s_dec;
+  __shared__ int s_inst;
+  Stats st{0, 0, 0, 0};
+  long long nv = 0, ns = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_inst = atomicAdd(b.next, 1);
+    __syncthreads();
+    const int i = s_inst;
+    __syncthreads();
+    if (i >= b.k) break;
+    batch_instance(g, b, b.order[i], s_f, s_c, s_flag, &s_dec, st, nv, ns);
+  }
+  block_flush(st.active, st.slots, st.pushes, st.relabels, b.stats, ST_ACTIVE);
+  block_flush(nv, ns, 0, 0, b.stats, ST_BFS_VERTS);
+}
+
 }  // namespace mf
 
 // ============================================================================================
@@ -556,6 +787,7 @@ constexpr float kDefaultRelabelRatio = 2.0f;
 constexpr int kMaxAdaptiveCycles = 1 << 14;  // cap of cycles per round in adaptive mode
 constexpr int kDefaultBfsFrontier = 4096;    // one-workgroup BFS up to this frontier size
 constexpr int kCycleChunk = 64;              // cycles launched between two worklist checks
+constexpr int kBatchGroupsPerCU = 4;         // resident workgroups per CU in the batched solve
 
 template <class T>
 static mf_status dmalloc(T **p, size_t count) {
@@ -1037,17 +1269,145 @@ static mf_status check_options(const mf_options &o) {
 }
 
 // Packed-instance solve used by the batch entry point.
+// Whole-instance-per-workgroup solve of an uploaded packed graph (BCSR).
+static mf_status run_batch_wg(mf_context *ctx, DevGraph &G, const int32_t *vbase, mf_algo algo,
+                              const mf_options &o, mf_result *res) {
+  hipStream_t st = ctx->stream;
+  const int32_t k = G.k;
+  const ClassConfig vc = class_config(MF_ALGO_VC, 0);
+  const ClassConfig cc = class_config(algo, o.tile_lanes);
+  // slot degree and class of every vertex (the same classes upload() stored on the device)
+  std::vector<int32_t> indeg((size_t)G.n, 0);
+  for (int64_t i = 0; i < G.m; ++i) indeg[(size_t)G.h_dst[(size_t)i]]++;
+  std::vector<int32_t> nver((size_t)k), pop((size_t)k * kNumClasses, 0),
+      bpop((size_t)k * kNumClasses, 0), order((size_t)k);
+  for (int32_t i = 0; i < k; ++i) {
+    nver[(size_t)i] = vbase[i + 1] - vbase[i];
+    for (int32_t v = vbase[i]; v < vbase[i + 1]; ++v) {
+      const int64_t deg = (int64_t)(G.h_off[(size_t)v + 1] - G.h_off[(size_t)v]) + indeg[(size_t)v];
+      bpop[(size_t)i * kNumClasses + class_of_degree(vc, deg)]++;
+      pop[(size_t)i * kNumClasses + (cc.nclass == 1 ? 0 : class_of_degree(cc, deg))]++;
+    }
+  }
+  for (int32_t i = 0; i < k; ++i) order[(size_t)i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t x, int32_t y) { return nver[(size_t)x] > nver[(size_t)y]; });
+  int *d_vbase = nullptr, *d_nver = nullptr, *d_pop = nullptr, *d_bpop = nullptr,
+      *d_order = nullptr, *d_next = nullptr, *d_status = nullptr;
+  auto cleanup = [&]() {
+    for (void *p : {(void *)d_vbase, (void *)d_nver, (void *)d_pop, (void *)d_bpop,
+                    (void *)d_order, (void *)d_next, (void *)d_status})
+      if (p) (void)hipFree(p);
+  };
+  mf_status rc;
+  if ((rc = dmalloc(&d_vbase, (size_t)k)) || (rc = dmalloc(&d_nver, (size_t)k)) ||
+      (rc = dmalloc(&d_pop, (size_t)k * kNumClasses)) ||
+      (rc = dmalloc(&d_bpop, (size_t)k * kNumClasses)) || (rc = dmalloc(&d_order, (size_t)k)) ||
+      (rc = dmalloc(&d_next, (size_t)1)) || (rc = dmalloc(&d_status, (size_t)k))) {
+    cleanup();
+    return rc;
+  }
+  bool ok =
+      hipMemcpyAsync(d_vbase, vbase, sizeof(int) * k, hipMemcpyHostToDevice, st) == hipSuccess &&
+      hipMemcpyAsync(d_nver, nver.data(), sizeof(int) * k, hipMemcpyHostToDevice, st) == hipSuccess &&
+      hipMemcpyAsync(d_pop, pop.data(), sizeof(int) * pop.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+      hipMemcpyAsync(d_bpop, bpop.data(), sizeof(int) * bpop.size(), hipMemcpyHostToDevice, st) == hipSuccess &&
+      hipMemcpyAsync(d_order, order.data(), sizeof(int) * k, hipMemcpyHostToDevice, st) == hipSuccess &&
+      hipStreamSynchronize(st) == hipSuccess;
+  if (!ok) {
+    cleanup();
+    return MF_ERR_DEVICE;
+  }
+  BatchArgs b;
+  memset(&b, 0, sizeof(b));
+  b.h = G.h;
+  b.e = G.e;
+  b.lists = G.lists;
+  b.N = G.n;
+  b.vbase = d_vbase;
+  b.nver = d_nver;
+  b.src = G.src;
+  b.snk = G.snk;
+  b.pop = d_pop;
+  b.bpop = d_bpop;
+  b.vcls = G.vcls;
+  b.order = d_order;
+  b.k = k;
+  b.next = d_next;
+  b.values = G.values;
+  b.status = d_status;
+  b.ovf = G.ctl + CTL_FLAGS + 1;
+  b.stats = G.stats;
+  b.nclass = cc.nclass;
+  for (int c = 0; c < kNumClasses; ++c) {
+    b.lanes[c] = cc.lanes[c];
+    b.blanes[c] = vc.lanes[c];
+  }
+  b.use_vcls = cc.nclass > 1;
+  b.ratio = o.relabel_time_ratio > 0 ? o.relabel_time_ratio : kDefaultRelabelRatio;
+  b.max_rounds = o.max_rounds > 0 ? (int)std::min<int64_t>(o.max_rounds, INT32_MAX) : 1000000;
+  b.max_cycles = o.cycles_per_relabel > 0 ? o.cycles_per_relabel : kMaxAdaptiveCycles;
+  if (o.cycles_per_relabel > 0) b.ratio = 1e30f;  // fixed cadence: only the cycle cap ends a round
+  const int nwg = std::max(1, std::min(k, kBatchGroupsPerCU * ctx->num_cu));
+  const double t0 = now_ms();
+  const int rgrid = std::max(1, (int)std::min<int64_t>((G.S + 255) / 256, 16 * ctx->num_cu));
+  k_reset_pairs<<<rgrid, 256, 0, st>>>(G.dc, G.cap0, G.S, 1);
+  k_reset_vertices<<<rgrid, 256, 0, st>>>(G.h, G.e, G.n, nullptr, 0);
+  if (hipMemsetAsync(G.stats, 0, sizeof(unsigned long long) * 16, st) != hipSuccess ||
+      hipMemsetAsync(G.ctl, 0, sizeof(int) * CTL_SIZE, st) != hipSuccess ||
+      hipMemsetAsync(d_next, 0, sizeof(int), st) != hipSuccess) {
+    cleanup();
+    return MF_ERR_DEVICE;
+  }
+  k_batch_wg<LayoutBCSR><<<nwg, kBlock, 0, st>>>(G.bcsr(), b);
+  std::vector<int> status((size_t)k);
+  unsigned long long hs[16];
+  int hctl[CTL_SIZE];
+  ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+  const double t1 = now_ms();
+  ok = ok &&
+       hipMemcpy(status.data(), d_status, sizeof(int) * k, hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(hs, G.stats, sizeof(hs), hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(hctl, G.ctl, sizeof(hctl), hipMemcpyDeviceToHost) == hipSuccess;
+  cleanup();
+  if (!ok) return MF_ERR_DEVICE;
+  mf_status result = MF_OK;
+  for (int32_t i = 0; i < k; ++i) {
+    if (status[(size_t)i] == 1 && result == MF_OK) result = MF_ERR_ITERATION_LIMIT;
+    if (status[(size_t)i] == 2) result = MF_ERR_INTERNAL;
+  }
+  if (hctl[CTL_FLAGS + 1]) result = MF_ERR_INTERNAL;  // worklist overflow
+  if (res) {
+    memset(res, 0, sizeof(*res));
+    res->solve_ms = t1 - t0;
+    res->kernel_ms = t1 - t0;
+    res->active_total = (int64_t)hs[ST_ACTIVE];
+    res->slots_scanned = (int64_t)hs[ST_SLOTS];
+    res->pushes = (int64_t)hs[ST_PUSHES];
+    res->relabels = (int64_t)hs[ST_RELABELS];
+    res->bfs_slots = (int64_t)hs[ST_BFS_SLOTS];
+    res->bytes_algorithmic = algorithmic_bytes_cycle(MF_LAYOUT_BCSR, res->active_total,
+                                                     res->slots_scanned, res->pushes, res->relabels);
+    res->status = result;
+  }
+  G.solved = result == MF_OK;
+  return result;
+}
+
 mf_status batch_solve_packed(mf_context *ctx, const mf_csr *packed, const int32_t *src,
-                             const int32_t *snk, int32_t k, mf_algo algo, mf_layout layout,
-                             const mf_options &o, int64_t *values, mf_result *res) {
+                             const int32_t *snk, const int32_t *vbase, int32_t k, mf_algo algo,
+                             mf_layout layout, const mf_options &o, int64_t *values,
+                             mf_result *res) {
   mf_status st = check_options(o);
   if (st != MF_OK) return st;
   HIP_TRY(hipSetDevice(ctx->device));
   DevGraph G;
+  const bool per_group = o.batch_mode == 0;
   const double t0 = now_ms();
-  st = upload(ctx, packed, src, snk, k, layout, G);
+  st = upload(ctx, packed, src, snk, k, per_group ? MF_LAYOUT_BCSR : layout, G);
   const double t1 = now_ms();
-  if (st == MF_OK) st = solve_graph(ctx, G, algo, o, res);
+  if (st == MF_OK)
+    st = per_group ? run_batch_wg(ctx, G, vbase, algo, o, res) : solve_graph(ctx, G, algo, o, res);
   if (res) res->upload_ms = t1 - t0;
   if (st == MF_OK && values) {
     std::vector<long long> v((size_t)k);
