@@ -13,6 +13,9 @@ Workloads
                       batched solve; compared with one-by-one single-instance solves
   serving             Poisson arrivals of mixed requests (RLG, Genrmf, bipartite; 1e3..1e6
                       vertices) through the continuous-batching server; p50/p99 latency
+  genrmf128           Genrmf a=b=128 (2,097,152 vertices), the size of the paper's Genrmf row
+  copy                1 GiB device-to-device copies: the achievable HBM rate, and the known-size
+                      kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters
 
 Every value is checked exactly: single instances against their committed values, batched and
 served results against solves of the same instances one by one.  Prints ONE JSON line (the
@@ -22,6 +25,7 @@ over the mean launch time, from HIP events around every launch on the solver's s
 bounded sample of the same workload).
 """
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -49,6 +53,9 @@ WORKLOADS = {
         desc="R-MAT scale 22 edge factor 16 a=.57 b=.19 c=.19 caps [1,1000] seed=1 "
              "(4,194,304 vertices, 65,244,556 arcs after dropping self-loops and duplicates)",
         gen=("gen_rmat", (22, 16, 0.57, 0.19, 0.19, 1000, 1)), value=18886759, algo="vc"),
+    "genrmf128": dict(
+        desc="Genrmf a=128 b=128 c1=1 c2=10000 seed=1 (2,097,152 vertices, 10,403,840 arcs)",
+        gen=("gen_genrmf", (128, 128, 1, 10000, 1)), value=81048332, algo="tc"),
     "rlg64": dict(
         desc="Washington RLG 64x64 r=10000 seed=1 (4,098 vertices, 12,224 arcs)",
         gen=("gen_washington_rlg", (64, 64, 10000, 1)), value=466088, algo="tc"),
@@ -98,22 +105,39 @@ def cpu_baseline_reference(mf):
                       f"{dt:.2f} s wall, pinned to core 0; value = residual slots / time"}
 
 
-def roofline_of(r):
-    """Roofline of the dominant kernel of a solve run with profile_kernels=1."""
-    if r.kernel_ms >= r.relabel_ms and r.profiled_launches:
-        kname = "k_cycle (push-relabel cycle)"
-        per_launch_bytes = r.bytes_algorithmic / r.profiled_launches
-        per_launch_ms = r.profiled_kernel_ms / r.profiled_launches
-        launches = r.profiled_launches
+def kernel_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload`, from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, scaled by a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
+
+
+def roofline_of(results, workload):
+    """Roofline of the dominant kernel over solves run with profile_kernels=1: algorithmic bytes
+    per launch over the mean launch time, both summed over every profiled solve."""
+    kernel_ms = sum(r.kernel_ms for r in results)
+    relabel_ms = sum(r.relabel_ms for r in results)
+    launches = sum(r.profiled_launches for r in results)
+    if kernel_ms >= relabel_ms and launches:
+        kname = "k_cycle"
+        per_launch_bytes = sum(r.bytes_algorithmic for r in results) / launches
+        per_launch_ms = sum(r.profiled_kernel_ms for r in results) / launches
     else:
-        kname = "k_bfs_wg + k_bfs_level (global relabel)"
-        per_launch_bytes = r.bytes_bfs / max(r.rounds, 1)
-        per_launch_ms = r.relabel_ms / max(r.rounds, 1)
-        launches = r.rounds
+        kname = "k_bfs_wg"  # global relabel; its time is taken per BFS
+        launches = sum(r.rounds for r in results)
+        per_launch_bytes = sum(r.bytes_bfs for r in results) / max(launches, 1)
+        per_launch_ms = relabel_ms / max(launches, 1)
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None, "bytes_per_launch": int(per_launch_bytes),
+            "traffic": kernel_traffic(kname, workload),
+            "bytes_per_launch": int(per_launch_bytes),
             "mean_launch_us": round(per_launch_ms * 1e3, 3), "launches": int(launches)}
 
 
@@ -145,14 +169,19 @@ def bench_single(args, mf):
     for _ in range(args.warmup):
         solve_checked()
     ctx.synchronize()
+    # every timed solve records start/end events in the dispatch packet of each cycle launch
     t0 = time.perf_counter()
-    results = [solve_checked() for _ in range(args.steps)]
+    results = [solve_checked(profile_kernels=1) for _ in range(args.steps)]
     ctx.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / args.steps
     slots = 2 * g.m
-    log(f"{algo_name}/{args.layout}: {ms_per_step:.2f} ms per solve")
-    roofline = roofline_of(solve_checked(profile_kernels=1))
+    roofline = roofline_of(results, args.workload)
+    t1 = time.perf_counter()
+    plain = [solve_checked() for _ in range(min(3, args.steps))]
+    plain_ms = 1e3 * (time.perf_counter() - t1) / len(plain)
+    log(f"{algo_name}/{args.layout}: {ms_per_step:.2f} ms per solve with launch events, "
+        f"{plain_ms:.2f} ms without")
 
     variants = {}
     if not args.no_variants:
@@ -176,6 +205,8 @@ def bench_single(args, mf):
     config = {"workload": W["desc"], "algo": algo_name, "layout": args.layout,
               "max_flow": W["value"], "residual_slots": slots, "global_relabels": last.rounds,
               "cycles": last.cycles, "pushes": last.pushes, "relabels": last.relabels,
+              "slots_scanned": last.slots_scanned,
+              "scanned_slots_per_s": round(last.slots_scanned / (last.solve_ms * 1e-3), 1),
               "device": dev_name, "variants_solve_ms": variants}
     return line_common(args, round(slots / (wall / args.steps), 1), "residual edges/s",
                        ms_per_step, config, roofline, cpu)
@@ -266,6 +297,42 @@ def bench_batched(args, mf):
 
 
 # ---------------------------------------------------------------------------------------------
+def bench_copy(args):
+    """hipMemcpy device-to-device of 1 GiB: one read and one write of every byte."""
+    hip = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib",
+                                   "libamdhip64.so"))
+    nbytes = 1 << 30
+
+    def ok(rc, what):
+        if rc != 0:
+            raise SystemExit(f"{what} failed with HIP error {rc}")
+
+    ok(hip.hipSetDevice(0), "hipSetDevice")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    ok(hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)), "hipMalloc")
+    ok(hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)), "hipMalloc")
+    ok(hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)), "hipMemset")
+    ok(hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+    t0 = 0.0
+    for step in range(args.warmup + args.steps):
+        if step == args.warmup:
+            ok(hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+            t0 = time.perf_counter()
+        ok(hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3), "hipMemcpy")  # device to device
+    ok(hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+    wall = time.perf_counter() - t0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    gbs = 2 * nbytes * args.steps / wall / 1e9
+    log(f"copy: {gbs:.1f} GB/s over {args.steps} copies of {nbytes} bytes")
+    config = {"workload": f"device-to-device copy of {nbytes} bytes (read + write)",
+              "bytes_per_copy": nbytes}
+    return line_common(args, round(gbs, 1), "GB/s", 1e3 * wall / args.steps, config,
+                       {"bound": "hbm", "kernel": "runtime copy", "achieved": round(gbs, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "traffic": 2 * nbytes}, None)
+
+
 def serving_requests(mf, n, seed):
     """Mixed requests, 1e3..1e6 vertices log-uniform, generator chosen uniformly."""
     rng = np.random.default_rng(seed)
@@ -344,7 +411,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="genrmf64",
-                    choices=sorted(WORKLOADS) + ["batched", "serving"])
+                    choices=sorted(WORKLOADS) + ["batched", "serving", "copy"])
     ap.add_argument("--algo", default=None, choices=["tc", "vc"])
     ap.add_argument("--layout", default="bcsr", choices=["bcsr", "rcsr", "bcsr2"])
     ap.add_argument("--cycles", type=int, default=0, help="push-relabel cycles per global relabel")
@@ -363,6 +430,9 @@ def main():
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "copy":
+        print(json.dumps(bench_copy(args)), flush=True)
+        return
     import wbpr_amd as mf
 
     if args.workload == "batched":

@@ -5,7 +5,8 @@ Every value is computed by the CPU oracle's Dinic and, where it finishes quickly
 against the C restatement of the reference serial solver and against the reference's own
 serial binary (oracle/_ref/maxflow-serial, built from the reference sources).  Disagreement
 aborts.  Run from the repository root:  python tests/golden/make_golden.py [--big]
-(--big adds the large benchmark instances; Genrmf 64x64 takes about a minute of Dinic).
+(--big adds the large benchmark instances; Genrmf 64x64 takes about a minute of Dinic,
+Genrmf 128x128 about half an hour).
 """
 import json
 import os
@@ -52,6 +53,7 @@ GENERATED = {
 BIG = {
     "genrmf64_s1": ("gen_genrmf", (64, 64, 1, 10000, 1)),
     "rmat22_s1": ("gen_rmat", (22, 16, 0.57, 0.19, 0.19, 1000, 1)),
+    "genrmf128_s1": ("gen_genrmf", (128, 128, 1, 10000, 1)),  # about 30 minutes of Dinic
 }
 SERIAL_MAX_N = 3000  # the serial solvers are O(n) per operation
 

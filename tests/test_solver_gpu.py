@@ -134,6 +134,13 @@ def test_rmat22_full_size(mf, orc, ctx, golden):
     solve_and_certify(mf, orc, ctx, g, "vc", "bcsr", value)
 
 
+@pytest.mark.parametrize("algo", ALGOS)
+def test_genrmf128_paper_size(mf, orc, ctx, golden, algo):
+    # the size of the paper's Genrmf row: 2,097,152 vertices, 10,403,840 arcs
+    g, value = generated_graph(mf, golden, "genrmf128_s1")
+    solve_and_certify(mf, orc, ctx, g, algo, "bcsr", value)
+
+
 def test_rcsr_reference_entry_point(mf, orc, ctx, golden):
     # preflow() + push_relabel() of the reference on its own RCSR host arrays
     g, value = fixture_graph(mf, golden, "m30.max")

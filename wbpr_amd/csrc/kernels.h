@@ -160,11 +160,22 @@ struct ListSet {
   int *ovf;
 };
 
+// a[i] for a run-time i without addressing the array: a chain of selects keeps a by-value
+// argument struct in registers (an indexed access would copy it to scratch memory).
+template <typename V, int N>
+__device__ __forceinline__ V pick(const V (&a)[N], int i) {
+  V v = a[0];
+#pragma unroll
+  for (int q = 1; q < N; ++q) v = (i == q) ? a[q] : v;
+  return v;
+}
+
 // Number of items of class c, clamped to the region capacity.  The counter was completed by an
 // earlier kernel or, inside the one-workgroup BFS, lives in LDS behind a workgroup barrier.
 __device__ __forceinline__ int list_count(const ListSet &L, int c) {
   const int k = L.cnt[c];
-  return k < L.cls_cap[c] ? k : L.cls_cap[c];
+  const int cap = pick(L.cls_cap, c);
+  return k < cap ? k : cap;
 }
 
 struct Dispatch {
@@ -179,7 +190,9 @@ __device__ __forceinline__ void append(const ListSet &L, int item, bool want, in
                                        int nclass) {
   const int lane = __lane_id();
   if (__ballot(want) == 0) return;
-  for (int c = 0; c < nclass; ++c) {
+#pragma unroll
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (c >= nclass) break;
     const bool mine = want && cls == c;
     const unsigned long long mask = __ballot(mine);
     if (mask == 0) continue;

@@ -21,6 +21,7 @@
 //             appended to the next worklist exactly once (see finish_vertex).
 // The push/relabel rule is the lock-free one of He & Hong that the reference uses: only the
 // owner of u lowers e(u) and r(u,*), so stale reads of other vertices' state are safe.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -246,7 +247,7 @@ __device__ __forceinline__ void discharge_block(const L &g, const CycleArgs &a, 
 template <int T, class L>
 __device__ __forceinline__ void cycle_class(const L &g, const CycleArgs &a, int c, int cnt, int b,
                                             int nblk, Stats &st) {
-  const int base = a.in.cls_off[c];
+  const int base = pick(a.in.cls_off, c);
   if constexpr (T <= 64) {
     constexpr int tpb = kBlock / T;
     const int tile = b * tpb + (int)threadIdx.x / T;
@@ -261,7 +262,9 @@ __device__ __forceinline__ void cycle_class(const L &g, const CycleArgs &a, int 
 // Block b of the grid serves degree class c when d.blk_begin[c] <= b < d.blk_begin[c+1].
 __device__ __forceinline__ int block_class(const Dispatch &d) {
   int c = 0;
-  while (c + 1 < d.nclass && (int)blockIdx.x >= d.blk_begin[c + 1]) ++c;
+#pragma unroll
+  for (int q = 1; q < kNumClasses; ++q)
+    if (q < d.nclass && (int)blockIdx.x >= d.blk_begin[q]) c = q;
   return c;
 }
 
@@ -271,11 +274,12 @@ __global__ __launch_bounds__(kBlock) void k_cycle(L g, CycleArgs a, Dispatch d) 
   // the following one
   if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses) a.clear_cnt[threadIdx.x] = 0;
   const int c = block_class(d);
-  const int nblk = d.blk_begin[c + 1] - d.blk_begin[c];
-  const int b = (int)blockIdx.x - d.blk_begin[c];
+  const int b0 = pick(d.blk_begin, c);
+  const int nblk = pick(d.blk_begin, c + 1) - b0;
+  const int b = (int)blockIdx.x - b0;
   const int cnt = list_count(a.in, c);
   Stats st{0, 0, 0, 0};
-  switch (d.lanes[c]) {
+  switch (pick(d.lanes, c)) {
     case 1: cycle_class<1>(g, a, c, cnt, b, nblk, st); break;
     case 8: cycle_class<8>(g, a, c, cnt, b, nblk, st); break;
     case 16: cycle_class<16>(g, a, c, cnt, b, nblk, st); break;
@@ -351,7 +355,7 @@ template <int T, class L>
 __device__ __forceinline__ void bfs_class(const L &g, const BfsArgs &a, const ListSet &in,
                                           const ListSet &out, int c, int cnt, int tile, int ntiles,
                                           int lane, int level, long long &nv, long long &ns) {
-  const int base = in.cls_off[c];
+  const int base = pick(in.cls_off, c);
   for (int i = tile; i < cnt; i += ntiles) {
     bfs_vertex<T>(g, a, out, in.items[base + i], lane, level, ns);
     if (lane == 0) ++nv;
@@ -414,18 +418,19 @@ __global__ void k_bfs_seed(BfsArgs a, const int *src, const int *snk, int k) {
 template <class L>
 __global__ __launch_bounds__(kBlock) void k_bfs_level(L g, BfsArgs a, Dispatch d, int level) {
   if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses)
-    a.fcnt[(level + 2) % 3][threadIdx.x] = 0;
+    pick(a.fcnt, (level + 2) % 3)[threadIdx.x] = 0;
   const int c = block_class(d);
-  const int nblk = d.blk_begin[c + 1] - d.blk_begin[c];
-  const int b = (int)blockIdx.x - d.blk_begin[c];
-  ListSet in = a.fr[level & 1];
-  in.cnt = a.fcnt[level % 3];
-  ListSet out = a.fr[(level + 1) & 1];
-  out.cnt = a.fcnt[(level + 1) % 3];
+  const int b0 = pick(d.blk_begin, c);
+  const int nblk = pick(d.blk_begin, c + 1) - b0;
+  const int b = (int)blockIdx.x - b0;
+  ListSet in = (level & 1) ? a.fr[1] : a.fr[0];
+  in.cnt = pick(a.fcnt, level % 3);
+  ListSet out = (level & 1) ? a.fr[0] : a.fr[1];
+  out.cnt = pick(a.fcnt, (level + 1) % 3);
   const int cnt = list_count(in, c);
   long long nv = 0, ns = 0;
-  bfs_dispatch(g, a, in, out, c, d.lanes[c], cnt, b * kBlock + (int)threadIdx.x, nblk * kBlock,
-               level, nv, ns);
+  bfs_dispatch(g, a, in, out, c, pick(d.lanes, c), cnt, b * kBlock + (int)threadIdx.x,
+               nblk * kBlock, level, nv, ns);
   block_flush(nv, ns, 0, 0, a.stats, ST_BFS_VERTS);
 }
 
@@ -444,7 +449,7 @@ __global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d,
   const int tid = (int)threadIdx.x;
   if (tid < 3 * kNumClasses) {
     const int q = tid / kNumClasses, c = tid % kNumClasses;
-    s_cnt[q][c] = (q == level % 3) ? a.fcnt[q][c] : 0;
+    s_cnt[q][c] = (q == level % 3) ? pick(a.fcnt, q)[c] : 0;
   }
   __syncthreads();
   while (true) {
@@ -457,21 +462,21 @@ __global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d,
     __syncthreads();
     const int size = s_size;
     if (size == 0 || size > fmax) break;
-    ListSet in = a.fr[level & 1];
+    ListSet in = (level & 1) ? a.fr[1] : a.fr[0];
     in.cnt = s_cnt[level % 3];
-    ListSet out = a.fr[(level + 1) & 1];
+    ListSet out = (level & 1) ? a.fr[0] : a.fr[1];
     out.cnt = s_cnt[(level + 1) % 3];
     for (int c = 0; c < d.nclass; ++c) {
       const int cnt = list_count(in, c);
       if (cnt == 0) continue;
-      bfs_dispatch(g, a, in, out, c, d.lanes[c], cnt, tid, kWgBlock, level, nv, ns);
+      bfs_dispatch(g, a, in, out, c, pick(d.lanes, c), cnt, tid, kWgBlock, level, nv, ns);
     }
     __syncthreads();
     ++level;
   }
   if (tid < 3 * kNumClasses) {
     const int q = tid / kNumClasses, c = tid % kNumClasses;
-    a.fcnt[q][c] = (q == level % 3) ? s_cnt[q][c] : 0;
+    pick(a.fcnt, q)[c] = (q == level % 3) ? s_cnt[q][c] : 0;
   }
   if (tid == 0) {
     a.state[0] = level;
@@ -567,7 +572,7 @@ struct BatchArgs {
 template <class L>
 __device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &ca, int c, int lanes,
                                                   int cnt, Stats &st) {
-  const int base = ca.in.cls_off[c];
+  const int base = pick(ca.in.cls_off, c);
   const int tid = (int)threadIdx.x;
   switch (lanes) {
 #define MF_BATCH_CASE(TT)                                                          \
@@ -588,26 +593,31 @@ __device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &c
 }
 
 template <class L>
-__device__ void batch_instance(const L &g, const BatchArgs &b, int inst,
+__device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, int inst,
                                int (*s_f)[kNumClasses], int (*s_c)[kNumClasses], int *s_flag,
                                int *s_dec, Stats &st, long long &nv, long long &ns) {
   const int tid = (int)threadIdx.x;
   const int vb = b.vbase[inst], n = b.nver[inst], s = b.src[inst], t = b.snk[inst];
   const int H = n;  // heights are distances inside the instance
-  ListSet sets[4];
-  for (int q = 0; q < 4; ++q) {
+  // frontier lists F0, F1 and cycle worklists S0, S1, each split into class regions; kept as
+  // four named values (an indexed local array would live in scratch memory)
+  auto make_set = [&](int q) {
+    ListSet ls;
     const int *pp = (q < 2 ? b.bpop : b.pop) + (size_t)inst * kNumClasses;
     const int nc = q < 2 ? kNumClasses : b.nclass;
-    sets[q].items = b.lists + (size_t)q * b.N;
+    ls.items = b.lists + (size_t)q * b.N;
     int acc = vb;
+#pragma unroll
     for (int c = 0; c < kNumClasses; ++c) {
-      sets[q].cls_off[c] = acc;
-      sets[q].cls_cap[c] = c < nc ? pp[c] : 0;
+      ls.cls_off[c] = acc;
+      ls.cls_cap[c] = c < nc ? pp[c] : 0;
       if (c < nc) acc += pp[c];
     }
-    sets[q].cnt = nullptr;
-    sets[q].ovf = b.ovf;
-  }
+    ls.cnt = nullptr;
+    ls.ovf = b.ovf;
+    return ls;
+  };
+  const ListSet F0 = make_set(0), F1 = make_set(1), S0 = make_set(2), S1 = make_set(3);
   BfsArgs ba;
   ba.h = b.h;
   ba.e = b.e;
@@ -616,10 +626,10 @@ __device__ void batch_instance(const L &g, const BatchArgs &b, int inst,
   ba.nclass = kNumClasses;
   ba.acls = b.use_vcls ? b.vcls : nullptr;
   ba.nclass_act = b.nclass;
-  ba.fr[0] = sets[0];
-  ba.fr[1] = sets[1];
+  ba.fr[0] = F0;
+  ba.fr[1] = F1;
   for (int q = 0; q < 3; ++q) ba.fcnt[q] = s_f[q];
-  ba.active = sets[2];
+  ba.active = S0;
   ba.active.cnt = s_c[0];
   ba.flags = s_flag;
   ba.state = nullptr;
@@ -665,7 +675,7 @@ __device__ void batch_instance(const L &g, const BatchArgs &b, int inst,
     if (tid == 0) {
       b.h[t] = 0;
       b.h[s] = H + 1;
-      ListSet f0 = sets[0];
+      ListSet f0 = F0;
       f0.cnt = s_f[0];
       append(f0, t, true, b.vcls[t], kNumClasses);
     }
@@ -681,13 +691,13 @@ __device__ void batch_instance(const L &g, const BatchArgs &b, int inst,
       const int size = *s_dec;
       __syncthreads();
       if (size == 0) break;
-      ListSet in = sets[level & 1];
+      ListSet in = (level & 1) ? F1 : F0;
       in.cnt = s_f[level % 3];
-      ListSet out = sets[(level + 1) & 1];
+      ListSet out = (level & 1) ? F0 : F1;
       out.cnt = s_f[(level + 1) % 3];
       for (int c = 0; c < kNumClasses; ++c) {
         const int cnt = list_count(in, c);
-        if (cnt) bfs_dispatch(g, ba, in, out, c, b.blanes[c], cnt, tid, kBlock, level, nv, ns);
+        if (cnt) bfs_dispatch(g, ba, in, out, c, pick(b.blanes, c), cnt, tid, kBlock, level, nv, ns);
       }
       __syncthreads();
     }
@@ -702,14 +712,14 @@ __device__ void batch_instance(const L &g, const BatchArgs &b, int inst,
     // ---- push-relabel cycles ----
     const long long t_cyc = wall_clock64();
     for (int q = 0; q < b.max_cycles; ++q) {
-      ca.in = sets[2 + (q & 1)];
+      ca.in = (q & 1) ? S1 : S0;
       ca.in.cnt = s_c[q % 3];
-      ca.out = sets[2 + ((q + 1) & 1)];
+      ca.out = (q & 1) ? S0 : S1;
       ca.out.cnt = s_c[(q + 1) % 3];
       if (tid < kNumClasses) s_c[(q + 2) % 3][tid] = 0;  // last read in cycle q - 1
       for (int c = 0; c < b.nclass; ++c) {
         const int cnt = list_count(ca.in, c);
-        if (cnt) batch_cycle_class(g, ca, c, b.lanes[c], cnt, st);
+        if (cnt) batch_cycle_class(g, ca, c, pick(b.lanes, c), cnt, st);
       }
       __syncthreads();
       if (tid == 0) {
@@ -1104,9 +1114,9 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   double kernel_ms = 0, relabel_ms = 0, prof_ms = 0;
   int64_t prof_launches = 0, cycles = 0, rounds = 0, bfs_levels = 0, bfs_grid_levels = 0;
   mf_status status = MF_OK;
-  if (o.profile_kernels && ctx->prof_ev.size() < (size_t)2 * kCycleChunk) {
+  if (o.profile_kernels && ctx->prof_ev.size() < (size_t)4 * kCycleChunk) {
     const size_t old = ctx->prof_ev.size();
-    ctx->prof_ev.resize((size_t)2 * kCycleChunk);
+    ctx->prof_ev.resize((size_t)4 * kCycleChunk);
     for (size_t q = old; q < ctx->prof_ev.size(); ++q) HIP_TRY(hipEventCreate(&ctx->prof_ev[q]));
   }
 
@@ -1177,37 +1187,53 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     //      empty (checked between chunks with one small read-back) ----
     HIP_TRY(hipEventRecord(e0, st));
     const double tc0 = now_ms();
-    int done = 0;
+    int done = 0, nchunk = 0, pending = 0;  // pending: launches of the previous profiled chunk
+    // elapsed times of a chunk's launches are read while the next chunk runs
+    auto collect = [&](int half, int count) -> hipError_t {
+      for (int q = 0; q < count; ++q) {
+        float kms = 0;
+        const size_t pe = (size_t)half * 2 * kCycleChunk + (size_t)2 * q;
+        hipError_t err = hipEventElapsedTime(&kms, ctx->prof_ev[pe], ctx->prof_ev[pe + 1]);
+        if (err != hipSuccess) return err;
+        prof_ms += kms;
+      }
+      prof_launches += count;
+      return hipSuccess;
+    };
     while (done < K) {
       const int chunk = std::min(kCycleChunk, K - done);
+      const int half = nchunk & 1;
       for (int q = done; q < done + chunk; ++q) {
         ca.in = sets[2 + (q & 1)];
         ca.in.cnt = ccnt[q % 3];
         ca.out = sets[2 + ((q + 1) & 1)];
         ca.out.cnt = ccnt[(q + 1) % 3];
         ca.clear_cnt = ccnt[(q + 2) % 3];
-        const size_t pe = (size_t)2 * (q - done);
-        if (o.profile_kernels) HIP_TRY(hipEventRecord(ctx->prof_ev[pe], st));
-        k_cycle<L><<<grid, kBlock, 0, st>>>(lay, ca, disp);
-        if (o.profile_kernels) HIP_TRY(hipEventRecord(ctx->prof_ev[pe + 1], st));
+        if (o.profile_kernels) {
+          // the events ride in the dispatch packet, so they time the kernel alone
+          const size_t pe = (size_t)half * 2 * kCycleChunk + (size_t)2 * (q - done);
+          void *args[] = {(void *)&lay, (void *)&ca, (void *)&disp};
+          HIP_TRY(hipExtLaunchKernel((const void *)k_cycle<L>, dim3(grid), dim3(kBlock), args, 0,
+                                     st, ctx->prof_ev[pe], ctx->prof_ev[pe + 1], 0));
+        } else {
+          k_cycle<L><<<grid, kBlock, 0, st>>>(lay, ca, disp);
+        }
       }
       HIP_TRY(hipGetLastError());
       done += chunk;
       HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_SIZE, hipMemcpyDeviceToHost, st));
-      HIP_TRY(hipStreamSynchronize(st));
       if (o.profile_kernels) {
-        for (int q = 0; q < chunk; ++q) {
-          float kms = 0;
-          HIP_TRY(hipEventElapsedTime(&kms, ctx->prof_ev[(size_t)2 * q], ctx->prof_ev[(size_t)2 * q + 1]));
-          prof_ms += kms;
-        }
-        prof_launches += chunk;
+        if (pending) HIP_TRY(collect(half ^ 1, pending));
+        pending = chunk;
       }
+      ++nchunk;
+      HIP_TRY(hipStreamSynchronize(st));
       int64_t left = 0;
       for (int c = 0; c < cfg.nclass; ++c) left += hstate[CTL_CCNT + (done % 3) * kNumClasses + c];
       if (left == 0) break;
       if (adaptive && now_ms() - tc0 >= ratio * last_relabel_ms) break;
     }
+    if (pending) HIP_TRY(collect((nchunk - 1) & 1, pending));
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipEventSynchronize(e1));
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
