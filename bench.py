@@ -158,7 +158,9 @@ def bench_single(args, mf):
     ctx = mf.Context(0)
     dev_name, ncu = ctx.device_info()
     log(f"device {dev_name}, {ncu} CUs; workload {W['desc']}")
-    dg = ctx.upload(g, layout)
+    t_up = time.perf_counter()
+    dg = ctx.upload(g, layout)  # host layout build + host-to-device copy
+    upload_ms = 1e3 * (time.perf_counter() - t_up)
 
     def solve_checked(**kw):
         r = dg.solve(algo, **{**opts, **kw})
@@ -205,7 +207,8 @@ def bench_single(args, mf):
     config = {"workload": W["desc"], "algo": algo_name, "layout": args.layout,
               "max_flow": W["value"], "residual_slots": slots, "global_relabels": last.rounds,
               "cycles": last.cycles, "pushes": last.pushes, "relabels": last.relabels,
-              "slots_scanned": last.slots_scanned,
+              "slots_scanned": last.slots_scanned, "upload_ms": round(upload_ms, 2),
+              "host_inclusive_residual_edges_per_s": round(slots / ((upload_ms + ms_per_step) * 1e-3), 1),
               "scanned_slots_per_s": round(last.slots_scanned / (last.solve_ms * 1e-3), 1),
               "device": dev_name, "variants_solve_ms": variants}
     return line_common(args, round(slots / (wall / args.steps), 1), "residual edges/s",
