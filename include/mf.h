@@ -87,14 +87,19 @@ typedef struct mf_options {
   int32_t tile_lanes;         /* VC lanes per vertex: 0 = by degree bucket, else 1/8/16/32/64/256 */
   int64_t max_rounds;         /* global-relabel rounds before MF_ERR_ITERATION_LIMIT; 0 = default */
   int32_t verbose;            /* 1: print one line per round like push_relabel.cu:121,194 */
-  int32_t profile_kernels;    /* 1: time every push-relabel launch with HIP events (adds overhead) */
+  int32_t profile_kernels;    /* N > 0: every N-th push-relabel launch carries start/end events in
+                                 its dispatch packet (1 = every launch; each timed launch costs a
+                                 cache flush, so sample for low overhead) */
   int32_t bfs_block_frontier; /* frontier size up to which the one-workgroup BFS runs; 0 = default */
   float relabel_time_ratio;   /* adaptive cadence: after a global relabel taking T ms, cycles run
                                  for about ratio * T ms; 0 = default (2) */
   int32_t batch_algo;         /* mf_batch_solve: 0 = TC (default), 1 = VC */
   int32_t batch_mode;         /* mf_batch_solve: 0 = one workgroup per instance (default),
                                  1 = all instances as one packed graph on the whole grid */
-  int32_t reserved[5];
+  int32_t bfs_direction;      /* global relabel: 0 = bottom-up levels while the frontier is large
+                                 (default), 1 = top-down levels only, 2 = every level that does not
+                                 run in the one-workgroup kernel is bottom-up */
+  int32_t reserved[4];
 } mf_options;
 
 typedef struct mf_result {
@@ -114,11 +119,12 @@ typedef struct mf_result {
   int64_t bytes_algorithmic;  /* algorithmic bytes of the push-relabel kernels */
   int64_t bytes_bfs;          /* algorithmic bytes of the global relabels */
   double profiled_kernel_ms;  /* with profile_kernels: summed per-launch event time */
-  int64_t profiled_launches;  /* with profile_kernels: number of timed launches */
+  int64_t profiled_launches;  /* with profile_kernels: number of timed launches (a sample of cycles) */
   int64_t bfs_levels;         /* BFS levels over all global relabels */
   int64_t bfs_grid_levels;    /* of which run as one grid-wide launch each */
   int32_t status;
-  int32_t reserved[7];
+  int32_t bfs_bottom_up_levels; /* BFS levels run bottom-up */
+  int32_t reserved[6];
 } mf_result;
 
 typedef struct mf_context mf_context;

@@ -91,11 +91,37 @@ def test_launch_geometry_stress(mf, orc, ctx, golden):
         for _ in range(24):
             opts = dict(cycles_per_relabel=int(rng.choice([1, 3, 16, 64, 1024])),
                         tile_lanes=int(rng.choice([0, 1, 8, 16, 32, 64, 256])),
-                        bfs_block_frontier=int(rng.choice([1, 8, 64, 4096])))
+                        bfs_block_frontier=int(rng.choice([1, 8, 64, 4096])),
+                        bfs_direction=int(rng.choice([0, 1, 2])))
             r = dg.solve(mf.ALGO_VC, **opts)
             assert r.value == value, opts
     finally:
         dg.close()
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("name", ["rlg32_s7", "genrmf16_s1", "rmat12_s3", "bip_3000_2000_s9",
+                                  "m50.max", "ww.max"])
+def test_bottom_up_bfs(mf, orc, ctx, golden, name, layout):
+    # every BFS level outside the one-workgroup kernel bottom-up (unreached vertices look for a
+    # residual arc into the current level), from the first level on
+    if name.endswith(".max"):
+        g, value = fixture_graph(mf, golden, name)
+    else:
+        g, value = generated_graph(mf, golden, name)
+    for algo in ALGOS:
+        r = solve_and_certify(mf, orc, ctx, g, algo, layout, value, bfs_direction=2,
+                              bfs_block_frontier=1)
+        assert r.bfs_bottom_up_levels > 0 or r.rounds == 1
+
+
+def test_bottom_up_levels_on_rmat18(mf, orc, ctx):
+    # the automatic direction choice switches to bottom-up on a power-law graph
+    g = mf.gen_rmat(18, 16, seed=2)
+    r = solve_and_certify(mf, orc, ctx, g, "vc", "bcsr", orc.dinic_maxflow(g))
+    assert r.bfs_bottom_up_levels > 0
+    top = solve_and_certify(mf, orc, ctx, g, "vc", "bcsr", r.value, bfs_direction=1)
+    assert top.bfs_bottom_up_levels == 0
 
 
 def test_resolve_same_graph(mf, ctx, golden):
@@ -170,6 +196,9 @@ def test_error_paths(mf, ctx, golden):
     dg = ctx.upload(g)
     with pytest.raises(mf.MFError) as ei:
         dg.solve(mf.ALGO_VC, tile_lanes=12)
+    assert ei.value.name == "MF_ERR_INVALID_ARGUMENT"
+    with pytest.raises(mf.MFError) as ei:
+        dg.solve(mf.ALGO_VC, bfs_direction=3)
     assert ei.value.name == "MF_ERR_INVALID_ARGUMENT"
     with pytest.raises(mf.MFError) as ei:
         dg.solve(mf.ALGO_VC, max_rounds=1, cycles_per_relabel=1)

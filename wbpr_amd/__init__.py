@@ -97,7 +97,8 @@ class _Options(C.Structure):
         ("relabel_time_ratio", C.c_float),
         ("batch_algo", C.c_int32),
         ("batch_mode", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("bfs_direction", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -123,7 +124,8 @@ class _Result(C.Structure):
         ("bfs_levels", C.c_int64),
         ("bfs_grid_levels", C.c_int64),
         ("status", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("bfs_bottom_up_levels", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 
@@ -410,6 +412,7 @@ class Result:
     profiled_launches: int
     bfs_levels: int
     bfs_grid_levels: int
+    bfs_bottom_up_levels: int
 
     @staticmethod
     def _from(r: _Result) -> "Result":

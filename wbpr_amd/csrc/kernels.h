@@ -156,9 +156,14 @@ struct ListSet {
   int *items;
   int cls_off[kNumClasses];
   int cls_cap[kNumClasses];
-  int *cnt;  // kNumClasses counters
+  int *cnt;     // kNumClasses counters, cnt[c * cstride]
+  int cstride;  // kCtlStride for counters in device memory, 1 for counters in LDS
   int *ovf;
 };
+
+// Counters in device memory sit one per 128-byte line: atomics on one line serialise (about
+// 11 ns each on gfx950), so classes must not share a line.
+constexpr int kCtlStride = 32;
 
 // a[i] for a run-time i without addressing the array: a chain of selects keeps a by-value
 // argument struct in registers (an indexed access would copy it to scratch memory).
@@ -173,7 +178,7 @@ __device__ __forceinline__ V pick(const V (&a)[N], int i) {
 // Number of items of class c, clamped to the region capacity.  The counter was completed by an
 // earlier kernel or, inside the one-workgroup BFS, lives in LDS behind a workgroup barrier.
 __device__ __forceinline__ int list_count(const ListSet &L, int c) {
-  const int k = L.cnt[c];
+  const int k = L.cnt[c * L.cstride];
   const int cap = pick(L.cls_cap, c);
   return k < cap ? k : cap;
 }
@@ -199,10 +204,82 @@ __device__ __forceinline__ void append(const ListSet &L, int item, bool want, in
     const int leader = __ffsll((long long)mask) - 1;
     const int rank = __popcll(mask & ((1ull << lane) - 1ull));
     int base = 0;
-    if (lane == leader) base = atomicAdd(&L.cnt[c], __popcll(mask));
+    if (lane == leader) base = atomicAdd(&L.cnt[c * L.cstride], __popcll(mask));
     base = __shfl(base, leader);
     if (mine) {
       if (base + rank < L.cls_cap[c]) L.items[L.cls_off[c] + base + rank] = item;
+      else *L.ovf = 1;
+    }
+  }
+}
+
+// ---- appends staged in LDS ------------------------------------------------------------------
+//
+// Grid-wide kernels stage their appends in LDS and reserve space in the device list once per
+// block and class at the end (stage_flush), instead of once per wave and class: a 10,000-block
+// launch would otherwise queue tens of thousands of atomics on the counter lines.  Appends that
+// do not fit the stage go to the device list directly.
+
+constexpr int kStageCap = 256;  // items per class a block stages
+
+struct Stage {
+  int cnt[kNumClasses];
+  int base[kNumClasses];
+  int items[kNumClasses][kStageCap];
+};
+
+__device__ __forceinline__ void stage_init(Stage &S) {
+  if (threadIdx.x < (unsigned)kNumClasses) S.cnt[threadIdx.x] = 0;
+  __syncthreads();
+}
+
+__device__ __forceinline__ void staged_append(Stage &S, const ListSet &L, int item, bool want,
+                                              int cls, int nclass) {
+  const int lane = __lane_id();
+  if (__ballot(want) == 0) return;
+#pragma unroll
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (c >= nclass) break;
+    const bool mine = want && cls == c;
+    const unsigned long long mask = __ballot(mine);
+    if (mask == 0) continue;
+    const int leader = __ffsll((long long)mask) - 1;
+    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&S.cnt[c], __popcll(mask));
+    base = __shfl(base, leader);
+    const bool fits = base + rank < kStageCap;
+    if (mine && fits) S.items[c][base + rank] = item;
+    const unsigned long long spill = __ballot(mine && !fits);
+    if (spill == 0) continue;
+    const int sl = __ffsll((long long)spill) - 1;
+    const int srank = __popcll(spill & ((1ull << lane) - 1ull));
+    int gbase = 0;
+    if (lane == sl) gbase = atomicAdd(&L.cnt[c * L.cstride], __popcll(spill));
+    gbase = __shfl(gbase, sl);
+    if (mine && !fits) {
+      if (gbase + srank < pick(L.cls_cap, c)) L.items[pick(L.cls_off, c) + gbase + srank] = item;
+      else *L.ovf = 1;
+    }
+  }
+}
+
+// Every thread of the block calls this once, after its last staged_append.
+__device__ __forceinline__ void stage_flush(Stage &S, const ListSet &L, int nclass) {
+  __syncthreads();
+  if (threadIdx.x < (unsigned)nclass) {
+    const int c = threadIdx.x;
+    const int k = min(S.cnt[c], kStageCap);
+    S.base[c] = k ? atomicAdd(&L.cnt[c * L.cstride], k) : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < kNumClasses; ++c) {
+    if (c >= nclass) break;
+    const int k = min(S.cnt[c], kStageCap);
+    const int base = S.base[c], cap = pick(L.cls_cap, c), off = pick(L.cls_off, c);
+    for (int i = (int)threadIdx.x; i < k; i += (int)blockDim.x) {
+      if (base + i < cap) L.items[off + base + i] = S.items[c][i];
       else *L.ovf = 1;
     }
   }
@@ -242,8 +319,34 @@ enum StatIdx {
   ST_BFS_VERTS = 4,
   ST_BFS_SLOTS = 5,
   ST_SOURCE_OUT = 6,
-  ST_COUNT = 8
+  ST_BU_VERTS = 8,  // bottom-up BFS levels: unreached vertices examined
+  ST_BU_SLOTS = 9,  //   and slots they scanned
+  ST_COUNT = 12
 };
+
+// Blocks [0, nblk) split into groups of max(1, nblk / cnt) blocks; group q takes items q,
+// q + ngroups, ... and its width = blocks * blockDim threads share each item.  Leftover blocks
+// get group >= cnt and no items.
+struct BlockGroups {
+  int group, ngroups, lane, width;
+};
+
+__device__ __forceinline__ BlockGroups block_groups(int b, int nblk, int cnt) {
+  const int gsz = max(1, nblk / max(cnt, 1));
+  BlockGroups bg;
+  bg.ngroups = max(1, nblk / gsz);
+  bg.group = b / gsz;
+  bg.lane = (b % gsz) * (int)blockDim.x + (int)threadIdx.x;
+  bg.width = gsz * (int)blockDim.x;
+  return bg;
+}
+
+// Statistics live in kStatSpread copies of kStatSlot counters, one 128-byte line per copy;
+// block b adds into copy b % kStatSpread and the host sums the copies.  (Atomics on one line
+// serialise at about 11 ns each: one copy for every block of a 10,000-block launch would cost
+// tenths of a millisecond.)
+constexpr int kStatSlot = 16;
+constexpr int kStatSpread = 64;
 
 __device__ __forceinline__ void block_flush(long long a, long long b, long long c, long long d,
                                             unsigned long long *out, int i0) {
@@ -264,7 +367,9 @@ __device__ __forceinline__ void block_flush(long long a, long long b, long long 
   if (threadIdx.x < 4) {
     long long s = 0;
     for (int k = 0; k < (int)(blockDim.x / 64); ++k) s += red[threadIdx.x][k];
-    if (s) atomicAdd(&out[i0 + threadIdx.x], (unsigned long long)s);
+    if (s)
+      atomicAdd(&out[(size_t)(blockIdx.x % kStatSpread) * kStatSlot + i0 + threadIdx.x],
+                (unsigned long long)s);
   }
 }
 

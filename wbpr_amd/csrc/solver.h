@@ -21,7 +21,7 @@ int class_of_degree(const ClassConfig &c, int64_t deg);
 int64_t algorithmic_bytes_cycle(mf_layout layout, int64_t active, int64_t slots, int64_t pushes,
                                 int64_t relabels);
 int64_t algorithmic_bytes_bfs(mf_layout layout, int32_t n, int64_t rounds, int64_t verts,
-                              int64_t slots);
+                              int64_t slots, int64_t bu_verts, int64_t bu_slots);
 
 const char *last_device_error();
 

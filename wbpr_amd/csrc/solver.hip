@@ -68,9 +68,10 @@ struct Cand {
   }
 };
 
-// Lane `first` of a group of `step` lanes scans slots first, first + step, ... of u, two slots
-// in flight (both slot records, then both neighbour heights) so the dependent loads overlap.
-template <class L>
+// Lane `first` of a group of `step` lanes scans slots first, first + step, ... of u, B slots in
+// flight: the B slot records are loaded together, then the B neighbour heights, so a lane waits
+// for two dependent loads per B slots instead of two per slot.
+template <int B, class L>
 __device__ __forceinline__ void scan_slots(const L &g, const int *h, int u, int first, int step,
                                            Cand &c, int &deg) {
   deg = 0;
@@ -78,24 +79,28 @@ __device__ __forceinline__ void scan_slots(const L &g, const int *h, int u, int 
   for (int seg = 0; seg < L::kSeg; ++seg) {
     const int2 r = g.range(u, seg);
     deg += r.y - r.x;
-    int j = r.x + first;
-    for (; j + step < r.y; j += 2 * step) {
-      int w0, r0, k0, w1, r1, k1;
-      g.scan(seg, j, w0, r0, k0);
-      g.scan(seg, j + step, w1, r1, k1);
-      const bool v0 = r0 > 0 && w0 != u, v1 = r1 > 0 && w1 != u;
-      const int h0 = v0 ? h[w0] : 0;
-      const int h1 = v1 ? h[w1] : 0;
-      if (v0) c.consider(h0, k0, w0, r0);
-      if (v1) c.consider(h1, k1, w1, r1);
-    }
-    if (j < r.y) {
-      int w0, r0, k0;
-      g.scan(seg, j, w0, r0, k0);
-      if (r0 > 0 && w0 != u) c.consider(h[w0], k0, w0, r0);
+    for (int j0 = r.x + first; j0 < r.y; j0 += B * step) {
+      int w[B], res[B], key[B], hw[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        w[b] = u;
+        res[b] = 0;
+        key[b] = 0;
+        if (j0 + b * step < r.y) g.scan(seg, j0 + b * step, w[b], res[b], key[b]);
+      }
+#pragma unroll
+      for (int b = 0; b < B; ++b) hw[b] = (res[b] > 0 && w[b] != u) ? h[w[b]] : 0;
+#pragma unroll
+      for (int b = 0; b < B; ++b)
+        if (res[b] > 0 && w[b] != u) c.consider(hw[b], key[b], w[b], res[b]);
     }
   }
 }
+
+// Slots in flight per lane: a thread-centric lane covers a whole adjacency list alone, a hub
+// block lane covers deg/256 slots; narrow tiles see few slots per lane.
+template <int T>
+constexpr int scan_batch() { return (T == 1 || T > 64) ? 8 : 4; }
 
 template <int T>
 __device__ __forceinline__ long long tile_scan(long long v) {
@@ -127,9 +132,9 @@ struct Plan {
 // Whichever of two atomics on one excess comes second sees the other's effect, so exactly one
 // of them appends.
 template <class L>
-__device__ __forceinline__ void finish_vertex(const L &g, const CycleArgs &a, int u, int hu,
-                                              const Plan &p, const Cand &mine, bool leader,
-                                              int deg, Stats &st) {
+__device__ __forceinline__ void finish_vertex(const L &g, const CycleArgs &a, Stage *S, int u,
+                                              int hu, const Plan &p, const Cand &mine,
+                                              bool leader, int deg, Stats &st) {
   bool want_u = false, want_w = false;
   if (leader) {
     st.active++;
@@ -158,9 +163,14 @@ __device__ __forceinline__ void finish_vertex(const L &g, const CycleArgs &a, in
     }
   }
   const int cu = a.vcls ? a.vcls[u] : 0;
-  append(a.out, u, want_u, cu, a.nclass);
   const int cw = (want_w && a.vcls) ? a.vcls[mine.w] : 0;
-  append(a.out, mine.w, want_w, cw, a.nclass);
+  if (S) {
+    staged_append(*S, a.out, u, want_u, cu, a.nclass);
+    staged_append(*S, a.out, mine.w, want_w, cw, a.nclass);
+  } else {
+    append(a.out, u, want_u, cu, a.nclass);
+    append(a.out, mine.w, want_w, cw, a.nclass);
+  }
 }
 
 __device__ __forceinline__ void plan_heights(unsigned long long best, int hu, int H, Plan &p) {
@@ -175,15 +185,15 @@ __device__ __forceinline__ long long admissible_residual(const Plan &p, const Ca
 
 // One discharge step of u by a tile of T lanes (T <= 64, inside one wave).
 template <int T, class L>
-__device__ __forceinline__ void discharge_tile(const L &g, const CycleArgs &a, int u, int lane,
-                                               Stats &st) {
+__device__ __forceinline__ void discharge_tile(const L &g, const CycleArgs &a, Stage *S, int u,
+                                               int lane, Stats &st) {
   const int hu = a.h[u];
   // e(u) grows while the tile runs: every lane uses lane 0's read (the value subtracted below)
   const int eu = __shfl(a.e[u], 0, T);
   if (hu >= a.H) return;  // became inactive after it was queued (all lanes read the same value)
   Cand c;
   int deg;
-  scan_slots(g, a.h, u, lane, T, c, deg);
+  scan_slots<scan_batch<T>()>(g, a.h, u, lane, T, c, deg);
   Plan p;
   const unsigned long long best = tile_min<T>(c.best);
   plan_heights(best, hu, a.H, p);
@@ -201,12 +211,13 @@ __device__ __forceinline__ void discharge_tile(const L &g, const CycleArgs &a, i
     p.d = winner ? total : 0;
     p.total = total;
   }
-  finish_vertex(g, a, u, hu, p, c, lane == 0, deg, st);
+  finish_vertex(g, a, S, u, hu, p, c, lane == 0, deg, st);
 }
 
 // One discharge step of a hub vertex by the whole block (kBlock lanes, LDS reductions).
 template <class L>
-__device__ __forceinline__ void discharge_block(const L &g, const CycleArgs &a, int u, Stats &st) {
+__device__ __forceinline__ void discharge_block(const L &g, const CycleArgs &a, Stage *S, int u,
+                                                Stats &st) {
   __shared__ unsigned long long red[kBlock / 64];
   __shared__ long long wsum[kBlock / 64];
   __shared__ int s_eu;
@@ -220,7 +231,7 @@ __device__ __forceinline__ void discharge_block(const L &g, const CycleArgs &a, 
   if (hu >= a.H) return;  // uniform over the block: only this block writes h[u]
   Cand c;
   int deg;
-  scan_slots(g, a.h, u, (int)threadIdx.x, kBlock, c, deg);
+  scan_slots<scan_batch<kBlock>()>(g, a.h, u, (int)threadIdx.x, kBlock, c, deg);
   const int wave = threadIdx.x / 64;
   unsigned long long best = tile_min<64>(c.best);
   if (__lane_id() == 0) red[wave] = best;
@@ -241,21 +252,22 @@ __device__ __forceinline__ void discharge_block(const L &g, const CycleArgs &a, 
   incl += before;
   p.d = (int)max(0LL, min(want, (long long)eu - (incl - want)));
   p.total = (int)min((long long)eu, sum);
-  finish_vertex(g, a, u, hu, p, c, threadIdx.x == 0, deg, st);
+  finish_vertex(g, a, S, u, hu, p, c, threadIdx.x == 0, deg, st);
 }
 
 template <int T, class L>
-__device__ __forceinline__ void cycle_class(const L &g, const CycleArgs &a, int c, int cnt, int b,
-                                            int nblk, Stats &st) {
+__device__ __forceinline__ void cycle_class(const L &g, const CycleArgs &a, Stage *S, int c,
+                                            int cnt, int b, int nblk, Stats &st) {
   const int base = pick(a.in.cls_off, c);
   if constexpr (T <= 64) {
     constexpr int tpb = kBlock / T;
     const int tile = b * tpb + (int)threadIdx.x / T;
     const int ntiles = nblk * tpb;
     const int lane = (int)threadIdx.x % T;
-    for (int i = tile; i < cnt; i += ntiles) discharge_tile<T>(g, a, a.in.items[base + i], lane, st);
+    for (int i = tile; i < cnt; i += ntiles)
+      discharge_tile<T>(g, a, S, a.in.items[base + i], lane, st);
   } else {
-    for (int i = b; i < cnt; i += nblk) discharge_block(g, a, a.in.items[base + i], st);
+    for (int i = b; i < cnt; i += nblk) discharge_block(g, a, S, a.in.items[base + i], st);
   }
 }
 
@@ -272,7 +284,10 @@ template <class L>
 __global__ __launch_bounds__(kBlock) void k_cycle(L g, CycleArgs a, Dispatch d) {
   // counters of the list two cycles ahead: last read by the previous launch, next written by
   // the following one
-  if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses) a.clear_cnt[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses)
+    a.clear_cnt[threadIdx.x * kCtlStride] = 0;
+  __shared__ Stage S;
+  stage_init(S);
   const int c = block_class(d);
   const int b0 = pick(d.blk_begin, c);
   const int nblk = pick(d.blk_begin, c + 1) - b0;
@@ -280,13 +295,14 @@ __global__ __launch_bounds__(kBlock) void k_cycle(L g, CycleArgs a, Dispatch d) 
   const int cnt = list_count(a.in, c);
   Stats st{0, 0, 0, 0};
   switch (pick(d.lanes, c)) {
-    case 1: cycle_class<1>(g, a, c, cnt, b, nblk, st); break;
-    case 8: cycle_class<8>(g, a, c, cnt, b, nblk, st); break;
-    case 16: cycle_class<16>(g, a, c, cnt, b, nblk, st); break;
-    case 32: cycle_class<32>(g, a, c, cnt, b, nblk, st); break;
-    case 64: cycle_class<64>(g, a, c, cnt, b, nblk, st); break;
-    default: cycle_class<kBlock>(g, a, c, cnt, b, nblk, st); break;
+    case 1: cycle_class<1>(g, a, &S, c, cnt, b, nblk, st); break;
+    case 8: cycle_class<8>(g, a, &S, c, cnt, b, nblk, st); break;
+    case 16: cycle_class<16>(g, a, &S, c, cnt, b, nblk, st); break;
+    case 32: cycle_class<32>(g, a, &S, c, cnt, b, nblk, st); break;
+    case 64: cycle_class<64>(g, a, &S, c, cnt, b, nblk, st); break;
+    default: cycle_class<kBlock>(g, a, &S, c, cnt, b, nblk, st); break;
   }
+  stage_flush(S, a.out, a.nclass);
   block_flush(st.active, st.slots, st.pushes, st.relabels, a.stats, ST_ACTIVE);
 }
 
@@ -308,26 +324,48 @@ struct BfsArgs {
   unsigned long long *stats;
 };
 
+// LDS stages of a grid-wide BFS launch (frontier and worklist appends), or none.
+struct BfsStages {
+  Stage *f;
+  Stage *a;
+};
+
+__device__ __forceinline__ void bfs_append(const BfsStages &S, const BfsArgs &a,
+                                           const ListSet &out, int w, bool claim, bool act) {
+  const int cw = (claim && a.vcls) ? a.vcls[w] : 0;
+  const int aw = (act && a.acls) ? a.acls[w] : 0;
+  if (S.f) {
+    staged_append(*S.f, out, w, claim, cw, a.nclass);
+    staged_append(*S.a, a.active, w, act, aw, a.nclass_act);
+  } else {
+    append(out, w, claim, cw, a.nclass);
+    append(a.active, w, act, aw, a.nclass_act);
+  }
+}
+
 // Slots one BFS lane keeps in flight: the four dependent steps (slot record, reverse residual,
 // neighbour height, claim) are each issued for the whole batch before waiting on the next.
 constexpr int kBfsBatch = 4;
 
-template <int T, class L>
-__device__ __forceinline__ void bfs_vertex(const L &g, const BfsArgs &a, const ListSet &out, int u,
-                                           int lane, int level, long long &ns) {
+// Lanes lane, lane + step, ... of a group of `step` lanes (a tile, a block or a group of
+// blocks: nothing here is shared inside the group) expand u's slots.
+template <class L>
+__device__ __forceinline__ void bfs_vertex(const L &g, const BfsArgs &a, const ListSet &out,
+                                           const BfsStages &S, int u, int lane, int step,
+                                           int level, long long &ns) {
   constexpr int B = kBfsBatch;
 #pragma unroll
   for (int seg = 0; seg < L::kSeg; ++seg) {
     const int2 r = g.range(u, seg);
     if (lane == 0) ns += r.y - r.x;
-    for (int base = r.x + lane; base < r.y; base += T * B) {
+    for (int base = r.x + lane; base < r.y; base += step * B) {
       int w[B], idx[B], rin[B], hw[B];
       bool claim[B];
 #pragma unroll
       for (int b = 0; b < B; ++b) {
         w[b] = -1;
         idx[b] = 0;
-        if (base + b * T < r.y) g.bfs_slot(seg, base + b * T, w[b], idx[b]);
+        if (base + b * step < r.y) g.bfs_slot(seg, base + b * step, w[b], idx[b]);
       }
 #pragma unroll
       for (int b = 0; b < B; ++b) rin[b] = (w[b] >= 0 && w[b] != u) ? g.bfs_rin(seg, idx[b]) : 0;
@@ -340,49 +378,54 @@ __device__ __forceinline__ void bfs_vertex(const L &g, const BfsArgs &a, const L
         else if (hw[b] == a.H) claim[b] = atomicCAS(&a.h[w[b]], a.H, level + 1) == a.H;
       }
 #pragma unroll
-      for (int b = 0; b < B; ++b) {
-        const bool act = claim[b] && a.e[w[b]] > 0;
-        const int cw = (claim[b] && a.vcls) ? a.vcls[w[b]] : 0;
-        const int aw = (act && a.acls) ? a.acls[w[b]] : 0;
-        append(out, w[b], claim[b], cw, a.nclass);
-        append(a.active, w[b], act, aw, a.nclass_act);
-      }
+      for (int b = 0; b < B; ++b) bfs_append(S, a, out, w[b], claim[b], claim[b] && a.e[w[b]] > 0);
     }
   }
 }
 
 template <int T, class L>
 __device__ __forceinline__ void bfs_class(const L &g, const BfsArgs &a, const ListSet &in,
-                                          const ListSet &out, int c, int cnt, int tile, int ntiles,
-                                          int lane, int level, long long &nv, long long &ns) {
+                                          const ListSet &out, const BfsStages &S, int c, int cnt,
+                                          int tile, int ntiles, int lane, int level, long long &nv,
+                                          long long &ns) {
   const int base = pick(in.cls_off, c);
   for (int i = tile; i < cnt; i += ntiles) {
-    bfs_vertex<T>(g, a, out, in.items[base + i], lane, level, ns);
+    bfs_vertex(g, a, out, S, in.items[base + i], lane, T, level, ns);
     if (lane == 0) ++nv;
   }
 }
 
 template <class L>
 __device__ __forceinline__ void bfs_dispatch(const L &g, const BfsArgs &a, const ListSet &in,
-                                             const ListSet &out, int c, int lanes, int cnt, int tid,
-                                             int nthreads, int level, long long &nv, long long &ns) {
+                                             const ListSet &out, const BfsStages &S, int c,
+                                             int lanes, int cnt, int tid, int nthreads, int level,
+                                             long long &nv, long long &ns) {
   // A BFS lane covers kBfsBatch slots per step, so a class processed with `lanes` lanes by the
   // cycle kernels uses lanes / kBfsBatch lanes per vertex here (same slots per step, four times
   // as many vertices in flight).
   switch (lanes) {
 #define MF_BFS_CASE(LANES, TT)                                                                \
   case LANES:                                                                                 \
-    bfs_class<TT>(g, a, in, out, c, cnt, tid / TT, nthreads / TT, tid % TT, level, nv, ns); \
+    bfs_class<TT>(g, a, in, out, S, c, cnt, tid / TT, nthreads / TT, tid % TT, level, nv, ns); \
     break;
     MF_BFS_CASE(1, 1)
     MF_BFS_CASE(8, 2)
     MF_BFS_CASE(16, 4)
     MF_BFS_CASE(32, 8)
     MF_BFS_CASE(64, 16)
-    default:
-      bfs_class<64>(g, a, in, out, c, cnt, tid / 64, nthreads / 64, tid % 64, level, nv, ns);
-      break;
 #undef MF_BFS_CASE
+    default: {
+      // hubs: a group of whole blocks per vertex, as many blocks as the class has per hub in
+      // the frontier (one hub of 200,000 slots would keep a single tile busy for milliseconds)
+      const int per = (int)blockDim.x;
+      const BlockGroups bg = block_groups(tid / per, nthreads / per, cnt);
+      const int base = pick(in.cls_off, c);
+      for (int i = bg.group; i < cnt; i += bg.ngroups) {
+        bfs_vertex(g, a, out, S, in.items[base + i], bg.lane, bg.width, level, ns);
+        if (bg.lane == 0) ++nv;
+      }
+      break;
+    }
   }
 }
 
@@ -392,11 +435,11 @@ __global__ void k_bfs_init(BfsArgs a, int *ccnt0, int *ccnt1) {
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < a.H; v += stride) a.h[v] = a.H;
   if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses) {
     const int c = threadIdx.x;
-    a.fcnt[0][c] = 0;
-    a.fcnt[1][c] = 0;
-    a.fcnt[2][c] = 0;
-    ccnt0[c] = 0;
-    ccnt1[c] = 0;
+    a.fcnt[0][c * kCtlStride] = 0;
+    a.fcnt[1][c * kCtlStride] = 0;
+    a.fcnt[2][c * kCtlStride] = 0;
+    ccnt0[c * kCtlStride] = 0;
+    ccnt1[c * kCtlStride] = 0;
     if (c == 0) a.flags[0] = 0;
   }
 }
@@ -418,7 +461,10 @@ __global__ void k_bfs_seed(BfsArgs a, const int *src, const int *snk, int k) {
 template <class L>
 __global__ __launch_bounds__(kBlock) void k_bfs_level(L g, BfsArgs a, Dispatch d, int level) {
   if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses)
-    pick(a.fcnt, (level + 2) % 3)[threadIdx.x] = 0;
+    pick(a.fcnt, (level + 2) % 3)[threadIdx.x * kCtlStride] = 0;
+  __shared__ Stage sf, sa;
+  stage_init(sf);
+  stage_init(sa);
   const int c = block_class(d);
   const int b0 = pick(d.blk_begin, c);
   const int nblk = pick(d.blk_begin, c + 1) - b0;
@@ -429,9 +475,124 @@ __global__ __launch_bounds__(kBlock) void k_bfs_level(L g, BfsArgs a, Dispatch d
   out.cnt = pick(a.fcnt, (level + 1) % 3);
   const int cnt = list_count(in, c);
   long long nv = 0, ns = 0;
-  bfs_dispatch(g, a, in, out, c, pick(d.lanes, c), cnt, b * kBlock + (int)threadIdx.x,
-               nblk * kBlock, level, nv, ns);
+  bfs_dispatch(g, a, in, out, BfsStages{&sf, &sa}, c, pick(d.lanes, c), cnt,
+               b * kBlock + (int)threadIdx.x, nblk * kBlock, level, nv, ns);
+  stage_flush(sf, out, a.nclass);
+  stage_flush(sa, a.active, a.nclass_act);
   block_flush(nv, ns, 0, 0, a.stats, ST_BFS_VERTS);
+}
+
+// ---- bottom-up levels (large frontiers) ----
+//
+// When the frontier holds a large part of the graph, a top-down level scans most slots of the
+// graph with two random gathers each (reverse residual, neighbour height) and claims by CAS.
+// Bottom-up, every still-unreached vertex w scans its own slots -- the residual of w -> u sits
+// in w's slot record, so one coalesced load and one gather per slot -- and stops at the first
+// residual arc into the current level.  The tile that owns w is its only writer: no CAS.
+
+// Vertices of nonzero degree grouped by degree class (static for a graph).
+struct VertexOrder {
+  const int *items;
+  int cls_off[kNumClasses];
+  int cls_cnt[kNumClasses];
+};
+
+// Lanes of a tile that reported a hit (the tile is a contiguous group of T lanes of the wave).
+template <int T>
+__device__ __forceinline__ bool tile_any(bool hit) {
+  const unsigned long long m = __ballot(hit);
+  if constexpr (T == 64) {
+    return m != 0;
+  } else {
+    const int shift = __lane_id() & ~(T - 1);
+    return ((m >> shift) & ((1ull << T) - 1ull)) != 0;
+  }
+}
+
+template <int T, class L>
+__device__ __forceinline__ void bu_vertex(const L &g, const BfsArgs &a, const ListSet &out,
+                                          const BfsStages &S, int w, int lane, int level,
+                                          long long &nv, long long &ns) {
+  constexpr int B = kBfsBatch;
+  // reached already, or a source (tile-uniform: one address).  Sources are not scanned here:
+  // the top-down levels check that no source reaches a sink
+  if (a.h[w] != a.H) return;
+  if (lane == 0) ++nv;
+  bool found = false;
+#pragma unroll
+  for (int seg = 0; seg < L::kSeg; ++seg) {
+    if (found) break;
+    const int2 r = g.range(w, seg);
+    for (int base = r.x + lane; base - lane < r.y; base += T * B) {
+      int u[B], res[B], key[B], hu[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        u[b] = w;
+        res[b] = 0;
+        if (base + b * T < r.y) g.scan(seg, base + b * T, u[b], res[b], key[b]);
+      }
+#pragma unroll
+      for (int b = 0; b < B; ++b) hu[b] = (res[b] > 0 && u[b] != w) ? a.h[u[b]] : -1;
+      bool hit = false;
+#pragma unroll
+      for (int b = 0; b < B; ++b) hit |= hu[b] == level;
+      if (lane == 0) ns += min(r.y - (base - lane), T * B);
+      if (tile_any<T>(hit)) {
+        found = true;
+        break;
+      }
+    }
+  }
+  const bool mine = found && lane == 0;
+  if (mine) a.h[w] = level + 1;
+  bfs_append(S, a, out, w, mine, mine && a.e[w] > 0);
+}
+
+template <int T, class L>
+__device__ __forceinline__ void bu_class(const L &g, const BfsArgs &a, const VertexOrder &vo,
+                                         const ListSet &out, const BfsStages &S, int c, int tile,
+                                         int ntiles, int lane, int level, long long &nv,
+                                         long long &ns) {
+  const int base = pick(vo.cls_off, c), cnt = pick(vo.cls_cnt, c);
+  for (int i = tile; i < cnt; i += ntiles)
+    bu_vertex<T>(g, a, out, S, vo.items[base + i], lane, level, nv, ns);
+}
+
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(L g, BfsArgs a, Dispatch d,
+                                                          VertexOrder vo, int level) {
+  if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses)
+    pick(a.fcnt, (level + 2) % 3)[threadIdx.x * kCtlStride] = 0;
+  __shared__ Stage sf, sa;
+  stage_init(sf);
+  stage_init(sa);
+  const BfsStages S{&sf, &sa};
+  const int c = block_class(d);
+  const int b0 = pick(d.blk_begin, c);
+  const int nblk = pick(d.blk_begin, c + 1) - b0;
+  const int tid = ((int)blockIdx.x - b0) * kBlock + (int)threadIdx.x;
+  const int nthreads = nblk * kBlock;
+  ListSet out = (level & 1) ? a.fr[0] : a.fr[1];
+  out.cnt = pick(a.fcnt, (level + 1) % 3);
+  long long nv = 0, ns = 0;
+  switch (pick(d.lanes, c)) {  // same lanes per vertex as the top-down levels
+#define MF_BU_CASE(LANES, TT)                                                                  \
+  case LANES:                                                                                  \
+    bu_class<TT>(g, a, vo, out, S, c, tid / TT, nthreads / TT, tid % TT, level, nv, ns);       \
+    break;
+    MF_BU_CASE(1, 1)
+    MF_BU_CASE(8, 2)
+    MF_BU_CASE(16, 4)
+    MF_BU_CASE(32, 8)
+    MF_BU_CASE(64, 16)
+    default:
+      bu_class<64>(g, a, vo, out, S, c, tid / 64, nthreads / 64, tid % 64, level, nv, ns);
+      break;
+#undef MF_BU_CASE
+  }
+  stage_flush(sf, out, a.nclass);
+  stage_flush(sa, a.active, a.nclass_act);
+  block_flush(nv, ns, 0, 0, a.stats, ST_BU_VERTS);
 }
 
 // All BFS levels inside one workgroup while the frontier holds at most fmax vertices: workgroup
@@ -449,7 +610,7 @@ __global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d,
   const int tid = (int)threadIdx.x;
   if (tid < 3 * kNumClasses) {
     const int q = tid / kNumClasses, c = tid % kNumClasses;
-    s_cnt[q][c] = (q == level % 3) ? pick(a.fcnt, q)[c] : 0;
+    s_cnt[q][c] = (q == level % 3) ? pick(a.fcnt, q)[c * kCtlStride] : 0;
   }
   __syncthreads();
   while (true) {
@@ -464,19 +625,22 @@ __global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d,
     if (size == 0 || size > fmax) break;
     ListSet in = (level & 1) ? a.fr[1] : a.fr[0];
     in.cnt = s_cnt[level % 3];
+    in.cstride = 1;
     ListSet out = (level & 1) ? a.fr[0] : a.fr[1];
     out.cnt = s_cnt[(level + 1) % 3];
+    out.cstride = 1;
     for (int c = 0; c < d.nclass; ++c) {
       const int cnt = list_count(in, c);
       if (cnt == 0) continue;
-      bfs_dispatch(g, a, in, out, c, pick(d.lanes, c), cnt, tid, kWgBlock, level, nv, ns);
+      bfs_dispatch(g, a, in, out, BfsStages{nullptr, nullptr}, c, pick(d.lanes, c), cnt, tid,
+                   kWgBlock, level, nv, ns);
     }
     __syncthreads();
     ++level;
   }
   if (tid < 3 * kNumClasses) {
     const int q = tid / kNumClasses, c = tid % kNumClasses;
-    pick(a.fcnt, q)[c] = (q == level % 3) ? s_cnt[q][c] : 0;
+    pick(a.fcnt, q)[c * kCtlStride] = (q == level % 3) ? s_cnt[q][c] : 0;
   }
   if (tid == 0) {
     a.state[0] = level;
@@ -512,12 +676,14 @@ template <class L>
 __global__ __launch_bounds__(kBlock) void k_preflow(L g, int *e, const int *src, int k,
                                                     unsigned long long *stats) {
   long long sum = 0;
-  for (int i = blockIdx.x; i < k; i += gridDim.x) {
+  // a group of blocks per source: a single source of high degree uses the whole grid
+  const BlockGroups bg = block_groups((int)blockIdx.x, (int)gridDim.x, k);
+  for (int i = bg.group; i < k; i += bg.ngroups) {
     const int s = src[i];
 #pragma unroll
     for (int seg = 0; seg < L::kSeg; ++seg) {
       const int2 r = g.range(s, seg);
-      for (int j = r.x + (int)threadIdx.x; j < r.y; j += kBlock) {
+      for (int j = r.x + bg.lane; j < r.y; j += bg.width) {
         int w, res, key;
         g.scan(seg, j, w, res, key);
         if (res > 0 && w != s) {
@@ -578,7 +744,7 @@ __device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &c
 #define MF_BATCH_CASE(TT)                                                          \
   case TT:                                                                         \
     for (int i = tid / TT; i < cnt; i += kBlock / TT)                               \
-      discharge_tile<TT>(g, ca, ca.in.items[base + i], tid % TT, st);              \
+      discharge_tile<TT>(g, ca, nullptr, ca.in.items[base + i], tid % TT, st);     \
     break;
     MF_BATCH_CASE(1)
     MF_BATCH_CASE(8)
@@ -587,7 +753,7 @@ __device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &c
     MF_BATCH_CASE(64)
 #undef MF_BATCH_CASE
     default:
-      for (int i = 0; i < cnt; ++i) discharge_block(g, ca, ca.in.items[base + i], st);
+      for (int i = 0; i < cnt; ++i) discharge_block(g, ca, nullptr, ca.in.items[base + i], st);
       break;
   }
 }
@@ -614,6 +780,7 @@ __device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, i
       if (c < nc) acc += pp[c];
     }
     ls.cnt = nullptr;
+    ls.cstride = 1;  // the workgroup's counters live in LDS
     ls.ovf = b.ovf;
     return ls;
   };
@@ -697,7 +864,9 @@ __device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, i
       out.cnt = s_f[(level + 1) % 3];
       for (int c = 0; c < kNumClasses; ++c) {
         const int cnt = list_count(in, c);
-        if (cnt) bfs_dispatch(g, ba, in, out, c, pick(b.blanes, c), cnt, tid, kBlock, level, nv, ns);
+        if (cnt)
+          bfs_dispatch(g, ba, in, out, BfsStages{nullptr, nullptr}, c, pick(b.blanes, c), cnt,
+                       tid, kBlock, level, nv, ns);
       }
       __syncthreads();
     }
@@ -796,7 +965,14 @@ static double now_ms() {
 constexpr float kDefaultRelabelRatio = 2.0f;
 constexpr int kMaxAdaptiveCycles = 1 << 14;  // cap of cycles per round in adaptive mode
 constexpr int kDefaultBfsFrontier = 4096;    // one-workgroup BFS up to this frontier size
-constexpr int kCycleChunk = 64;              // cycles launched between two worklist checks
+constexpr int kCycleChunk = 64;              // most cycles launched between two worklist checks
+constexpr int kFirstChunk = 8;               // first chunk, before a cycle has been timed
+// Direction of the grid-wide BFS levels (Beamer's rule): bottom-up once the frontier's slots
+// exceed 1/kBottomUpAlpha of the slots not yet explored, top-down again once the frontier holds
+// fewer than 1/kBottomUpBeta of the nonzero-degree vertices; never below kBottomUpMinVertices.
+constexpr int64_t kBottomUpAlpha = 14;
+constexpr int64_t kBottomUpBeta = 24;
+constexpr int64_t kBottomUpMinVertices = 1 << 16;
 constexpr int kBatchGroupsPerCU = 4;         // resident workgroups per CU in the batched solve
 
 template <class T>
@@ -849,7 +1025,7 @@ static Dispatch make_dispatch(const ClassConfig &c, const int64_t *pop, int num_
       const int64_t tpb = kBlock / c.lanes[k];
       nb = std::min<int64_t>((pop[k] + tpb - 1) / tpb, (int64_t)8 * num_cu);
     } else {
-      nb = std::min<int64_t>(pop[k], (int64_t)2 * num_cu);
+      nb = std::min<int64_t>(pop[k], (int64_t)8 * num_cu);
     }
     b += (int)std::max<int64_t>(nb, 1);
   }
@@ -869,11 +1045,15 @@ int64_t algorithmic_bytes_cycle(mf_layout layout, int64_t active, int64_t slots,
 }
 
 int64_t algorithmic_bytes_bfs(mf_layout layout, int32_t n, int64_t rounds, int64_t verts,
-                              int64_t slots) {
+                              int64_t slots, int64_t bu_verts, int64_t bu_slots) {
   // per round: height initialisation 4n; per frontier vertex: item 4 + offsets 8;
-  // per slot: slot record + reverse residual 4 + neighbour height 4
+  // per slot: slot record + reverse residual 4 + neighbour height 4.
+  // bottom-up: per unreached vertex examined: height 4 + offsets 8; per slot: slot record +
+  // neighbour height 4
   const int64_t per_slot = layout == MF_LAYOUT_BCSR ? 16 : (layout == MF_LAYOUT_BCSR2 ? 20 : 14);
-  return rounds * 4 * (int64_t)n + verts * 12 + slots * per_slot;
+  const int64_t bu_slot = layout == MF_LAYOUT_BCSR ? 12 : (layout == MF_LAYOUT_BCSR2 ? 16 : 14);
+  return rounds * 4 * (int64_t)n + verts * 12 + slots * per_slot + bu_verts * 12 +
+         bu_slots * bu_slot;
 }
 
 }  // namespace mf
@@ -916,6 +1096,12 @@ struct DevGraph {
   unsigned long long *stats = nullptr;
   unsigned char *vcls = nullptr;  // VC degree class per vertex
   int64_t vc_pop[kNumClasses] = {0};
+  int *vorder = nullptr;           // nonzero-degree vertices grouped by VC class
+  VertexOrder vo{};                // vorder's class regions
+  int64_t bu_pop[kNumClasses] = {0};
+  int64_t nonzero = 0;             // vertices of nonzero degree
+  int64_t class_slots[kNumClasses] = {0};  // residual slots of the vertices of each VC class
+  int64_t max_src_deg = 0;         // largest residual degree of a source
   // host copies for result read-back
   std::vector<int32_t> h_off, h_dst, h_cap, h_fslot, h_src, h_snk;
   bool solved = false;
@@ -925,21 +1111,74 @@ struct DevGraph {
   LayoutRCSR rcsr() const { return LayoutRCSR{off, fdc, bcf, roff, rdi, (int)m}; }
 };
 
-// control block (int offsets)
+// control block (int offsets); every counter on a 128-byte line of its own (kCtlStride)
 enum {
-  CTL_FCNT = 0,                           // 3 x kNumClasses frontier counters
-  CTL_CCNT = CTL_FCNT + 3 * kNumClasses,  // 3 x kNumClasses worklist counters
-  CTL_FLAGS = CTL_CCNT + 3 * kNumClasses, // [0] source reached, [1] list overflow
-  CTL_STATE = CTL_FLAGS + 2,              // [0] level, [1] frontier size
-  CTL_SIZE = CTL_STATE + 4
+  CTL_FCNT = 0,                                        // 3 x kNumClasses frontier counters
+  CTL_CCNT = CTL_FCNT + 3 * kNumClasses * kCtlStride,  // 3 x kNumClasses worklist counters
+  CTL_FLAGS = CTL_CCNT + 3 * kNumClasses * kCtlStride, // [0] source reached, [1] list overflow
+  CTL_STATE = CTL_FLAGS + kCtlStride,                  // [0] level, [1] frontier size
+  CTL_SIZE = CTL_STATE + kCtlStride
 };
+
+// counter c of triple-buffer slot q of the frontier (CTL_FCNT) or worklist (CTL_CCNT) set
+constexpr int ctl_cnt(int set, int q, int c) { return set + (q * kNumClasses + c) * kCtlStride; }
 
 static void free_dev(DevGraph *g) {
   void *ptrs[] = {g->off, g->dc,  g->rev, g->dst,    g->cr,    g->fdc,   g->bcf,
                   g->roff, g->rdi, g->cap0, g->src,   g->snk,   g->values, g->h,
-                  g->e,   g->lists, g->ctl, g->stats, g->vcls};
+                  g->e,   g->lists, g->ctl, g->stats, g->vcls, g->vorder};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+}
+
+// Sums the spread statistics copies (see block_flush).
+static hipError_t read_stats(const unsigned long long *dev, unsigned long long out[kStatSlot]) {
+  std::vector<unsigned long long> all((size_t)kStatSlot * kStatSpread);
+  hipError_t err = hipMemcpy(all.data(), dev, sizeof(unsigned long long) * all.size(),
+                             hipMemcpyDeviceToHost);
+  for (int i = 0; i < kStatSlot; ++i) {
+    out[i] = 0;
+    for (int q = 0; q < kStatSpread; ++q) out[i] += all[(size_t)q * kStatSlot + i];
+  }
+  return err;
+}
+
+// Degree classes of every vertex (VC buckets) and the class-grouped order of the vertices of
+// nonzero degree that bottom-up BFS levels walk.  degree counts residual slots.
+static mf_status set_vertex_classes(DevGraph &G, const std::vector<int64_t> &degree,
+                                    hipStream_t st) {
+  const int n = G.n;
+  mf_status rc;
+  if ((rc = dmalloc(&G.vcls, (size_t)n)) || (rc = dmalloc(&G.vorder, (size_t)std::max(n, 1))))
+    return rc;
+  const ClassConfig vc = class_config(MF_ALGO_VC, 0);
+  std::vector<unsigned char> cls((size_t)n);
+  for (int q = 0; q < kNumClasses; ++q) G.vc_pop[q] = G.bu_pop[q] = 0;
+  for (int q = 0; q < kNumClasses; ++q) G.class_slots[q] = 0;
+  for (int32_t v = 0; v < n; ++v) {
+    const int q = class_of_degree(vc, degree[(size_t)v]);
+    cls[(size_t)v] = (unsigned char)q;
+    G.vc_pop[q]++;
+    G.class_slots[q] += degree[(size_t)v];
+    if (degree[(size_t)v] > 0) G.bu_pop[q]++;
+  }
+  G.nonzero = 0;
+  int fill[kNumClasses];
+  for (int q = 0; q < kNumClasses; ++q) {
+    G.vo.cls_off[q] = fill[q] = (int)G.nonzero;
+    G.vo.cls_cnt[q] = (int)G.bu_pop[q];
+    G.nonzero += G.bu_pop[q];
+  }
+  std::vector<int> order((size_t)G.nonzero);
+  for (int32_t v = 0; v < n; ++v)
+    if (degree[(size_t)v] > 0) order[(size_t)fill[cls[(size_t)v]]++] = v;
+  G.vo.items = G.vorder;
+  HIP_TRY(hipMemcpyAsync(G.vcls, cls.data(), (size_t)n, hipMemcpyHostToDevice, st));
+  if (!order.empty())
+    HIP_TRY(hipMemcpyAsync(G.vorder, order.data(), sizeof(int) * order.size(),
+                           hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return MF_OK;
 }
 
 // Uploads a (packed) graph: c is the union CSR, src/snk the k terminal pairs.
@@ -1013,20 +1252,12 @@ static mf_status upload(mf_context *ctx, const mf_csr *c, const int32_t *src, co
   }
   if ((rc = dmalloc(&G.h, (size_t)c->n)) || (rc = dmalloc(&G.e, (size_t)c->n)) ||
       (rc = dmalloc(&G.lists, (size_t)4 * c->n)) || (rc = dmalloc(&G.ctl, (size_t)CTL_SIZE)) ||
-      (rc = dmalloc(&G.stats, (size_t)16)) || (rc = dmalloc(&G.vcls, (size_t)c->n)) ||
-      (rc = dmalloc(&G.src, (size_t)k)) || (rc = dmalloc(&G.snk, (size_t)k)) ||
-      (rc = dmalloc(&G.values, (size_t)k)))
+      (rc = dmalloc(&G.stats, (size_t)kStatSlot * kStatSpread)) || (rc = dmalloc(&G.src, (size_t)k)) ||
+      (rc = dmalloc(&G.snk, (size_t)k)) || (rc = dmalloc(&G.values, (size_t)k)))
     return rc;
-  // VC degree classes
-  const ClassConfig vc = class_config(MF_ALGO_VC, 0);
-  std::vector<unsigned char> cls((size_t)c->n);
-  for (int q = 0; q < kNumClasses; ++q) G.vc_pop[q] = 0;
-  for (int32_t v = 0; v < c->n; ++v) {
-    const int q = class_of_degree(vc, degree[(size_t)v]);
-    cls[(size_t)v] = (unsigned char)q;
-    G.vc_pop[q]++;
-  }
-  HIP_TRY(hipMemcpyAsync(G.vcls, cls.data(), c->n, hipMemcpyHostToDevice, st));
+  if ((rc = set_vertex_classes(G, degree, st)) != MF_OK) return rc;
+  G.max_src_deg = 0;
+  for (int32_t i = 0; i < k; ++i) G.max_src_deg = std::max(G.max_src_deg, degree[(size_t)src[i]]);
   HIP_TRY(hipMemcpyAsync(G.src, src, sizeof(int) * k, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(G.snk, snk, sizeof(int) * k, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemsetAsync(G.ctl, 0, sizeof(int) * CTL_SIZE, st));
@@ -1051,6 +1282,11 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   const ClassConfig bcfg = class_config(MF_ALGO_VC, 0);
   const Dispatch bdisp = make_dispatch(bcfg, G.vc_pop, ctx->num_cu);
   const int bgrid = bdisp.blk_begin[bcfg.nclass];
+  // bottom-up levels walk the nonzero-degree vertices in class order
+  const Dispatch budisp = make_dispatch(bcfg, G.bu_pop, ctx->num_cu);
+  const int bugrid = budisp.blk_begin[bcfg.nclass];
+  const bool bu_allowed = o.bfs_direction == 0 && G.nonzero >= kBottomUpMinVertices;
+  const bool bu_always = o.bfs_direction == 2;
 
   // list sets: F0, F1 (BFS frontier), S0, S1 (cycle worklists); class regions by population
   ListSet sets[4];
@@ -1065,12 +1301,13 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       if (c < nc) acc += pp[c];
     }
     sets[q].cnt = nullptr;
+    sets[q].cstride = kCtlStride;
     sets[q].ovf = G.ctl + CTL_FLAGS + 1;
   }
   int *fcnt[3], *ccnt[3];
   for (int q = 0; q < 3; ++q) {
-    fcnt[q] = G.ctl + CTL_FCNT + q * kNumClasses;
-    ccnt[q] = G.ctl + CTL_CCNT + q * kNumClasses;
+    fcnt[q] = G.ctl + ctl_cnt(CTL_FCNT, q, 0);
+    ccnt[q] = G.ctl + ctl_cnt(CTL_CCNT, q, 0);
   }
 
   BfsArgs ba;
@@ -1104,6 +1341,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   const int K = adaptive ? kMaxAdaptiveCycles : o.cycles_per_relabel;
   const double ratio = o.relabel_time_ratio > 0 ? o.relabel_time_ratio : kDefaultRelabelRatio;
   double last_relabel_ms = 0;
+  double cyc_est_ms = 0;  // wall time per cycle launch in the last chunk
   const int64_t max_rounds = o.max_rounds > 0 ? o.max_rounds : (int64_t)100000000;
   const int vgrid = std::max(1, std::min((n + 255) / 256, 16 * ctx->num_cu));
   const int kgrid = std::max(1, std::min((G.k + 255) / 256, 16 * ctx->num_cu));
@@ -1112,7 +1350,8 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   double kernel_ms = 0, relabel_ms = 0, prof_ms = 0;
-  int64_t prof_launches = 0, cycles = 0, rounds = 0, bfs_levels = 0, bfs_grid_levels = 0;
+  int64_t prof_launches = 0, cycles = 0, rounds = 0, bfs_levels = 0, bfs_grid_levels = 0,
+          bfs_bu_levels = 0;
   mf_status status = MF_OK;
   if (o.profile_kernels && ctx->prof_ev.size() < (size_t)4 * kCycleChunk) {
     const size_t old = ctx->prof_ev.size();
@@ -1127,9 +1366,11 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   else k_reset_pairs<<<rgrid, 256, 0, st>>>(G.fdc, G.cap0, G.m, 1);
   k_reset_vertices<<<rgrid, 256, 0, st>>>(G.h, G.e, n, G.layout == MF_LAYOUT_RCSR ? G.bcf : nullptr,
                                           G.m);
-  HIP_TRY(hipMemsetAsync(G.stats, 0, sizeof(unsigned long long) * 16, st));
+  HIP_TRY(hipMemsetAsync(G.stats, 0, sizeof(unsigned long long) * kStatSlot * kStatSpread, st));
   HIP_TRY(hipMemsetAsync(G.ctl, 0, sizeof(int) * CTL_SIZE, st));
-  k_preflow<L><<<std::min(G.k, 16 * ctx->num_cu), kBlock, 0, st>>>(lay, G.e, G.src, G.k, G.stats);
+  const int pgrid = (int)std::min<int64_t>(16 * ctx->num_cu,
+                                           std::max<int64_t>(G.k, (G.max_src_deg + 1023) / 1024));
+  k_preflow<L><<<pgrid, kBlock, 0, st>>>(lay, G.e, G.src, G.k, G.stats);
   HIP_TRY(hipGetLastError());
 
   int hstate[CTL_SIZE];
@@ -1143,6 +1384,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     k_bfs_init<<<vgrid, 256, 0, st>>>(ba, ccnt[0], ccnt[1]);
     k_bfs_seed<<<kgrid, 256, 0, st>>>(ba, G.src, G.snk, G.k);
     int level = 0, levels_total = 0;
+    double explored = 0;  // slots of the frontiers expanded by grid-wide levels
     for (;;) {
       k_bfs_wg<L><<<1, kWgBlock, 0, st>>>(lay, ba, bdisp, level, fmax);
       HIP_TRY(hipGetLastError());
@@ -1151,15 +1393,34 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       level = hstate[CTL_STATE];
       int size = hstate[CTL_STATE + 1];
       if (size == 0) break;
+      bool bottom_up = false;
       for (;;) {  // frontier too large for one workgroup: one launch per level
-        k_bfs_level<L><<<bgrid, kBlock, 0, st>>>(lay, ba, bdisp, level);
+        if (bu_allowed) {
+          // frontier slots estimated from its class counts and the classes' mean degrees
+          double mf = 0;
+          for (int c = 0; c < bcfg.nclass; ++c) {
+            const int cnt = hstate[ctl_cnt(CTL_FCNT, level % 3, c)];
+            if (G.vc_pop[c]) mf += (double)cnt * (double)G.class_slots[c] / (double)G.vc_pop[c];
+          }
+          const double mu = std::max(0.0, (double)G.S - explored);
+          if (!bottom_up && mf * kBottomUpAlpha > mu) bottom_up = true;
+          else if (bottom_up && (int64_t)size * kBottomUpBeta < G.nonzero) bottom_up = false;
+          explored += mf;
+        }
+        bottom_up = bottom_up || bu_always;
+        if (bottom_up) {
+          k_bfs_bottom_up<L><<<bugrid, kBlock, 0, st>>>(lay, ba, budisp, G.vo, level);
+          ++bfs_bu_levels;
+        } else {
+          k_bfs_level<L><<<bgrid, kBlock, 0, st>>>(lay, ba, bdisp, level);
+        }
         HIP_TRY(hipGetLastError());
         ++level;
         ++bfs_grid_levels;
         HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_SIZE, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         size = 0;
-        for (int c = 0; c < bcfg.nclass; ++c) size += hstate[CTL_FCNT + (level % 3) * kNumClasses + c];
+        for (int c = 0; c < bcfg.nclass; ++c) size += hstate[ctl_cnt(CTL_FCNT, level % 3, c)];
         if (size <= fmax_low) break;
       }
       if (size == 0) break;
@@ -1178,7 +1439,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       break;
     }
     int64_t active = 0;
-    for (int c = 0; c < cfg.nclass; ++c) active += hstate[CTL_CCNT + c];
+    for (int c = 0; c < cfg.nclass; ++c) active += hstate[ctl_cnt(CTL_CCNT, 0, c)];
     if (o.verbose)
       printf("round %lld: relabel levels %d, active vertices %lld\n", (long long)rounds,
              levels_total, (long long)active);
@@ -1187,8 +1448,8 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     //      empty (checked between chunks with one small read-back) ----
     HIP_TRY(hipEventRecord(e0, st));
     const double tc0 = now_ms();
-    int done = 0, nchunk = 0, pending = 0;  // pending: launches of the previous profiled chunk
-    // elapsed times of a chunk's launches are read while the next chunk runs
+    int done = 0, nchunk = 0, pending = 0;  // pending: timed launches of the previous chunk
+    // elapsed times of a chunk's timed launches are read while the next chunk runs
     auto collect = [&](int half, int count) -> hipError_t {
       for (int q = 0; q < count; ++q) {
         float kms = 0;
@@ -1201,17 +1462,29 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       return hipSuccess;
     };
     while (done < K) {
-      const int chunk = std::min(kCycleChunk, K - done);
+      // chunk length: what the rest of the time budget allows at the last measured cycle time
+      // (one read-back per chunk; short chunks when cycles are long, up to kCycleChunk)
+      int chunk = kCycleChunk;
+      if (adaptive) {
+        const double left_ms = ratio * last_relabel_ms - (now_ms() - tc0);
+        if (done > 0 && left_ms <= 0) break;
+        chunk = cyc_est_ms > 0
+                    ? (int)std::min<double>(kCycleChunk, std::max(1.0, left_ms / cyc_est_ms))
+                    : kFirstChunk;
+      }
+      chunk = std::min(chunk, K - done);
       const int half = nchunk & 1;
+      const double tk0 = now_ms();
+      int timed = 0;
       for (int q = done; q < done + chunk; ++q) {
         ca.in = sets[2 + (q & 1)];
         ca.in.cnt = ccnt[q % 3];
         ca.out = sets[2 + ((q + 1) & 1)];
         ca.out.cnt = ccnt[(q + 1) % 3];
         ca.clear_cnt = ccnt[(q + 2) % 3];
-        if (o.profile_kernels) {
-          // the events ride in the dispatch packet, so they time the kernel alone
-          const size_t pe = (size_t)half * 2 * kCycleChunk + (size_t)2 * (q - done);
+        if (o.profile_kernels > 0 && q % o.profile_kernels == 0) {
+          // every profile_kernels-th launch carries start/end events in its dispatch packet
+          const size_t pe = (size_t)half * 2 * kCycleChunk + (size_t)2 * timed++;
           void *args[] = {(void *)&lay, (void *)&ca, (void *)&disp};
           HIP_TRY(hipExtLaunchKernel((const void *)k_cycle<L>, dim3(grid), dim3(kBlock), args, 0,
                                      st, ctx->prof_ev[pe], ctx->prof_ev[pe + 1], 0));
@@ -1222,16 +1495,14 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       HIP_TRY(hipGetLastError());
       done += chunk;
       HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_SIZE, hipMemcpyDeviceToHost, st));
-      if (o.profile_kernels) {
-        if (pending) HIP_TRY(collect(half ^ 1, pending));
-        pending = chunk;
-      }
+      if (pending) HIP_TRY(collect(half ^ 1, pending));
+      pending = timed;
       ++nchunk;
       HIP_TRY(hipStreamSynchronize(st));
+      cyc_est_ms = (now_ms() - tk0) / chunk;
       int64_t left = 0;
-      for (int c = 0; c < cfg.nclass; ++c) left += hstate[CTL_CCNT + (done % 3) * kNumClasses + c];
+      for (int c = 0; c < cfg.nclass; ++c) left += hstate[ctl_cnt(CTL_CCNT, done % 3, c)];
       if (left == 0) break;
-      if (adaptive && now_ms() - tc0 >= ratio * last_relabel_ms) break;
     }
     if (pending) HIP_TRY(collect((nchunk - 1) & 1, pending));
     HIP_TRY(hipEventRecord(e1, st));
@@ -1245,10 +1516,10 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
   const double t1 = now_ms();
-  unsigned long long hs[16];
+  unsigned long long hs[kStatSlot];
   long long v0 = 0;
   HIP_TRY(hipMemcpy(&v0, G.values, sizeof(long long), hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(hs, G.stats, sizeof(hs), hipMemcpyDeviceToHost));
+  HIP_TRY(read_stats(G.stats, hs));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (res) {
@@ -1267,7 +1538,10 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     res->bfs_slots = (int64_t)hs[ST_BFS_SLOTS];
     res->bytes_algorithmic = algorithmic_bytes_cycle(G.layout, res->active_total,
                                                      res->slots_scanned, res->pushes, res->relabels);
-    res->bytes_bfs = algorithmic_bytes_bfs(G.layout, n, rounds, (int64_t)hs[ST_BFS_VERTS], res->bfs_slots);
+    res->bytes_bfs = algorithmic_bytes_bfs(G.layout, n, rounds, (int64_t)hs[ST_BFS_VERTS],
+                                           res->bfs_slots, (int64_t)hs[ST_BU_VERTS],
+                                           (int64_t)hs[ST_BU_SLOTS]);
+    res->bfs_bottom_up_levels = (int32_t)bfs_bu_levels;
     res->profiled_kernel_ms = prof_ms;
     res->profiled_launches = prof_launches;
     res->bfs_levels = bfs_levels;
@@ -1290,6 +1564,8 @@ static mf_status check_options(const mf_options &o) {
       o.tile_lanes != 32 && o.tile_lanes != 64 && o.tile_lanes != kBlock)
     return MF_ERR_INVALID_ARGUMENT;
   if (o.cycles_per_relabel < 0 || o.max_rounds < 0 || o.bfs_block_frontier < 0)
+    return MF_ERR_INVALID_ARGUMENT;
+  if (o.bfs_direction < 0 || o.bfs_direction > 2 || o.profile_kernels < 0)
     return MF_ERR_INVALID_ARGUMENT;
   return MF_OK;
 }
@@ -1379,7 +1655,7 @@ static mf_status run_batch_wg(mf_context *ctx, DevGraph &G, const int32_t *vbase
   const int rgrid = std::max(1, (int)std::min<int64_t>((G.S + 255) / 256, 16 * ctx->num_cu));
   k_reset_pairs<<<rgrid, 256, 0, st>>>(G.dc, G.cap0, G.S, 1);
   k_reset_vertices<<<rgrid, 256, 0, st>>>(G.h, G.e, G.n, nullptr, 0);
-  if (hipMemsetAsync(G.stats, 0, sizeof(unsigned long long) * 16, st) != hipSuccess ||
+  if (hipMemsetAsync(G.stats, 0, sizeof(unsigned long long) * kStatSlot * kStatSpread, st) != hipSuccess ||
       hipMemsetAsync(G.ctl, 0, sizeof(int) * CTL_SIZE, st) != hipSuccess ||
       hipMemsetAsync(d_next, 0, sizeof(int), st) != hipSuccess) {
     cleanup();
@@ -1387,13 +1663,13 @@ static mf_status run_batch_wg(mf_context *ctx, DevGraph &G, const int32_t *vbase
   }
   k_batch_wg<LayoutBCSR><<<nwg, kBlock, 0, st>>>(G.bcsr(), b);
   std::vector<int> status((size_t)k);
-  unsigned long long hs[16];
+  unsigned long long hs[kStatSlot];
   int hctl[CTL_SIZE];
   ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
   const double t1 = now_ms();
   ok = ok &&
        hipMemcpy(status.data(), d_status, sizeof(int) * k, hipMemcpyDeviceToHost) == hipSuccess &&
-       hipMemcpy(hs, G.stats, sizeof(hs), hipMemcpyDeviceToHost) == hipSuccess &&
+       read_stats(G.stats, hs) == hipSuccess &&
        hipMemcpy(hctl, G.ctl, sizeof(hctl), hipMemcpyDeviceToHost) == hipSuccess;
   cleanup();
   if (!ok) return MF_ERR_DEVICE;
@@ -1670,25 +1946,20 @@ mf_status mf_rcsr_push_relabel(mf_context *ctx, int algo_type, int V, int E, int
       (st = dmalloc(&G.rdi, (size_t)E)) || (st = dmalloc(&G.cap0, (size_t)E)) ||
       (st = dmalloc(&G.h, (size_t)V)) || (st = dmalloc(&G.e, (size_t)V)) ||
       (st = dmalloc(&G.lists, (size_t)4 * V)) || (st = dmalloc(&G.ctl, (size_t)CTL_SIZE)) ||
-      (st = dmalloc(&G.stats, (size_t)16)) || (st = dmalloc(&G.vcls, (size_t)V)) ||
-      (st = dmalloc(&G.src, (size_t)1)) || (st = dmalloc(&G.snk, (size_t)1)) ||
-      (st = dmalloc(&G.values, (size_t)1)))
+      (st = dmalloc(&G.stats, (size_t)kStatSlot * kStatSpread)) || (st = dmalloc(&G.src, (size_t)1)) ||
+      (st = dmalloc(&G.snk, (size_t)1)) || (st = dmalloc(&G.values, (size_t)1)))
     return fail(st);
-  const ClassConfig vc = class_config(MF_ALGO_VC, 0);
-  std::vector<unsigned char> cls((size_t)V);
-  for (int32_t v = 0; v < V; ++v) {
-    const int64_t deg = (int64_t)(cpu_offsets[v + 1] - cpu_offsets[v]) +
+  std::vector<int64_t> degree((size_t)V);
+  for (int32_t v = 0; v < V; ++v)
+    degree[(size_t)v] = (int64_t)(cpu_offsets[v + 1] - cpu_offsets[v]) +
                         (cpu_roffsets[v + 1] - cpu_roffsets[v]);
-    const int q = class_of_degree(vc, deg);
-    cls[(size_t)v] = (unsigned char)q;
-    G.vc_pop[q]++;
-  }
+  if ((st = set_vertex_classes(G, degree, s)) != MF_OK) return fail(st);
+  G.max_src_deg = degree[(size_t)source];
   bool ok = hipMemcpyAsync(G.off, cpu_offsets, sizeof(int) * (V + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(G.fdc, fdc.data(), sizeof(int2) * E, hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(G.roff, cpu_roffsets, sizeof(int) * (V + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(G.rdi, rdi.data(), sizeof(int2) * E, hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(G.cap0, cpu_capacities, sizeof(int) * E, hipMemcpyHostToDevice, s) == hipSuccess &&
-            hipMemcpyAsync(G.vcls, cls.data(), (size_t)V, hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(G.src, &source, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(G.snk, &sink, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
             hipStreamSynchronize(s) == hipSuccess;
