@@ -174,20 +174,29 @@ def bench_single(args, mf):
     for _ in range(args.warmup):
         solve_checked()
     ctx.synchronize()
+    # launch parameters from the autotuner (trial solves of this graph), before the timed solves
+    tuned, trials = {}, 0
+    if not args.no_autotune:
+        tuned, tune_ms, trials = dg.autotune(algo, **opts)
+        tuned = {k: v for k, v in tuned.items() if k not in ("batch_algo", "batch_mode")}
+        log(f"autotune: {trials} trial solves, best {tune_ms:.2f} ms with {tuned}")
     # every 16th cycle launch of the timed solves carries start/end events in its dispatch
     # packet (a timed launch costs a cache flush, so only a sample is timed)
     t0 = time.perf_counter()
-    results = [solve_checked(profile_kernels=PROFILE_EVERY) for _ in range(args.steps)]
+    results = [solve_checked(profile_kernels=PROFILE_EVERY, **tuned) for _ in range(args.steps)]
     ctx.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / args.steps
     slots = 2 * g.m
     roofline = roofline_of(results, args.workload)
     t1 = time.perf_counter()
-    plain = [solve_checked() for _ in range(min(3, args.steps))]
+    plain = [solve_checked(**tuned) for _ in range(min(3, args.steps))]
     plain_ms = 1e3 * (time.perf_counter() - t1) / len(plain)
+    t1 = time.perf_counter()
+    dflt = [solve_checked() for _ in range(min(3, args.steps))]
+    default_ms = 1e3 * (time.perf_counter() - t1) / len(dflt)
     log(f"{algo_name}/{args.layout}: {ms_per_step:.2f} ms per solve with launch events, "
-        f"{plain_ms:.2f} ms without")
+        f"{plain_ms:.2f} ms without, {default_ms:.2f} ms with default options")
 
     variants = {}
     if not args.no_variants:
@@ -212,6 +221,9 @@ def bench_single(args, mf):
               "max_flow": W["value"], "residual_slots": slots, "global_relabels": last.rounds,
               "cycles": last.cycles, "pushes": last.pushes, "relabels": last.relabels,
               "slots_scanned": last.slots_scanned, "upload_ms": round(upload_ms, 2),
+              "autotuned_options": tuned, "autotune_trials": trials,
+              "untimed_ms_per_solve": round(plain_ms, 3),
+              "default_options_ms_per_solve": round(default_ms, 3),
               "host_inclusive_residual_edges_per_s": round(slots / ((upload_ms + ms_per_step) * 1e-3), 1),
               "scanned_slots_per_s": round(last.slots_scanned / (last.solve_ms * 1e-3), 1),
               "device": dev_name, "variants_solve_ms": variants}
@@ -262,11 +274,19 @@ def bench_batched(args, mf):
     ctx = mf.Context(0)
     dev_name, ncu = ctx.device_info()
     algo_mode = 1 if args.algo == "vc" else 0  # batch_algo: 0 TC, 1 VC
+    tuned, trials = {"batch_algo": algo_mode}, 0
+    if not args.no_autotune:
+        t1 = time.perf_counter()
+        tuned, tune_ms, trials = ctx.batch_autotune(graphs, sample=args.tune_sample,
+                                                    batch_algo=algo_mode)
+        log(f"batch autotune: {trials} trial batches of {args.tune_sample} instances in "
+            f"{time.perf_counter() - t1:.1f} s, best {tune_ms:.2f} ms with {tuned}")
+        algo_mode = tuned["batch_algo"]
     runs = []
     values = None
     for step in range(args.warmup + args.steps):
         t1 = time.perf_counter()
-        vals, res = ctx.batch_solve(graphs, batch_algo=algo_mode)
+        vals, res = ctx.batch_solve(graphs, **tuned)
         wall = time.perf_counter() - t1
         if values is not None and not np.array_equal(values, vals):
             raise SystemExit("batched values changed between runs")
@@ -298,7 +318,8 @@ def bench_batched(args, mf):
               "batched_solves_per_s": round(batched_sps, 2),
               "one_by_one_solves_per_s": round(one_sps, 2),
               "one_by_one_sample": len(per), "speedup_vs_one_by_one": round(batched_sps / one_sps, 2),
-              "global_relabels": runs[-1].rounds, "cycles": runs[-1].cycles, "device": dev_name}
+              "autotuned_options": tuned, "autotune_trials": trials,
+              "autotune_sample": args.tune_sample, "device": dev_name}
     return line_common(args, round(batched_sps, 2), "solves/s", solve_ms, config,
                        None, cpu)
 
@@ -424,6 +445,8 @@ def main():
     ap.add_argument("--cycles", type=int, default=0, help="push-relabel cycles per global relabel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch parameters")
+    ap.add_argument("--tune-sample", type=int, default=256, help="batched: autotune sample")
     ap.add_argument("--instances", type=int, default=4096, help="batched: number of instances")
     ap.add_argument("--one-by-one", type=int, default=256, help="batched: one-by-one sample size")
     ap.add_argument("--cpu-sample", type=int, default=64, help="batched: CPU baseline sample")

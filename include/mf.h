@@ -99,7 +99,11 @@ typedef struct mf_options {
   int32_t bfs_direction;      /* global relabel: 0 = bottom-up levels while the frontier is large
                                  (default), 1 = top-down levels only, 2 = every level that does not
                                  run in the one-workgroup kernel is bottom-up */
-  int32_t reserved[4];
+  uint16_t class_lanes[5];    /* VC with tile_lanes 0: lanes per vertex for the degree classes
+                                 <=8, <=16, <=32, <=1024, larger slots; 0 = default (8, 16, 32, 64,
+                                 256); allowed 1, 2, 4, 8, 16, 32, 64, 256 (256 = whole block) */
+  uint16_t pad0;
+  int32_t reserved[1];
 } mf_options;
 
 typedef struct mf_result {
@@ -208,6 +212,14 @@ mf_status mf_graph_destroy(mf_context *ctx, mf_graph *g);
 mf_status mf_solve(mf_context *ctx, const mf_csr *g, mf_algo algo, mf_layout layout,
                    const mf_options *opts, mf_result *res, int32_t *flow_out);
 
+/* Autotuner (no reference counterpart): trial solves of the resident graph g with candidate tile
+ * widths (class_lanes, VC only), relabel cadences (relabel_time_ratio) and one-workgroup BFS
+ * limits (bfs_block_frontier), starting from *base (NULL = defaults), by coordinate descent.
+ * *best receives the fastest options, *best_ms their solve time, *trials the number of solves run
+ * (either may be NULL).  Every trial must produce the same flow value, else MF_ERR_INTERNAL. */
+mf_status mf_autotune(mf_context *ctx, mf_graph *g, mf_algo algo, const mf_options *base,
+                      mf_options *best, double *best_ms, int32_t *trials);
+
 /* Drop-in for the reference host->solver interface on RCSR arrays:
  * preflow() (maxflow-cuda/src/preflow.cu:3-31) followed by push_relabel()
  * (maxflow-cuda/src/push_relabel.cu:35-221), declared at maxflow-cuda/include/parallel_graph.cuh:44-54.
@@ -229,6 +241,12 @@ mf_status mf_rcsr_push_relabel(mf_context *ctx, int algo_type, int V, int E, int
  * TC (0, default) or VC (1) cycles.  values[k] receives each value. */
 mf_status mf_batch_solve(mf_context *ctx, const mf_csr *inst, int32_t k, const mf_options *opts,
                          int64_t *values, mf_result *total);
+/* Autotuner of the batched solve: trial batched solves of a sample of at most `sample` instances
+ * (the largest ones, which set the tail of a persistent launch, and an even spread of the rest)
+ * over batch_algo and relabel_time_ratio; *best receives the fastest options. */
+mf_status mf_batch_autotune(mf_context *ctx, const mf_csr *inst, int32_t k, int32_t sample,
+                            const mf_options *base, mf_options *best, double *best_ms,
+                            int32_t *trials);
 
 /* ---- serving (no reference counterpart) ----
  * A request loop over a device-resident arena of hbm_budget_bytes.  Submitted graphs are copied

@@ -115,3 +115,16 @@ def test_server_budget(mf, ctx, golden):
         assert srv.stats()["rejected"] == 1
     finally:
         srv.close()
+
+
+def test_batch_autotune(mf, orc, ctx):
+    rng = np.random.default_rng(44)
+    graphs = []
+    for i in range(48):
+        L, R = (int(round(np.exp(rng.uniform(np.log(64), np.log(4096))))) for _ in range(2))
+        graphs.append(mf.gen_bipartite(L, R, 8.0, seed=5000 + i))
+    expect = [orc.dinic_maxflow(g) for g in graphs]
+    best, ms, trials = ctx.batch_autotune(graphs, sample=16)
+    assert trials >= 8 and ms > 0 and best["batch_algo"] in (0, 1)
+    values, _ = ctx.batch_solve(graphs, **best)
+    assert values.tolist() == expect

@@ -239,3 +239,41 @@ def test_cli_snap_with_terminals(golden):
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "implementation is 4, 4" in r.stdout
+
+
+LANE_PRESETS = [[8, 16, 32, 64, 256], [2, 4, 8, 32, 256], [4, 8, 16, 64, 256], [1, 2, 8, 32, 256],
+                [2, 2, 2, 2, 2]]
+
+
+@pytest.mark.parametrize("lanes", LANE_PRESETS)
+@pytest.mark.parametrize("name", ["genrmf16_s1", "rmat12_s3", "bip_3000_2000_s9"])
+def test_class_lanes(mf, orc, ctx, golden, name, lanes):
+    # tile widths per degree class (the autotuner's search space), narrow ones included
+    g, value = generated_graph(mf, golden, name)
+    solve_and_certify(mf, orc, ctx, g, "vc", "bcsr", value, class_lanes=lanes)
+
+
+def test_class_lanes_rejected(mf, ctx, golden):
+    g, _ = generated_graph(mf, golden, "genrmf8_s7")
+    dg = ctx.upload(g, mf.LAYOUT_BCSR)
+    try:
+        with pytest.raises(mf.MFError) as ei:
+            dg.solve(mf.ALGO_VC, class_lanes=[8, 16, 3, 64, 256])
+        assert ei.value.name == "MF_ERR_INVALID_ARGUMENT"
+    finally:
+        dg.close()
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+def test_autotune(mf, orc, ctx, golden, algo):
+    g, value = generated_graph(mf, golden, "rmat12_s3")
+    dg = ctx.upload(g, mf.LAYOUT_BCSR)
+    try:
+        best, ms, trials = dg.autotune(mf.ALGOS[algo])
+        assert trials >= 8 and ms > 0
+        assert best["relabel_time_ratio"] > 0
+        r = dg.solve(mf.ALGOS[algo], **best)
+        assert r.value == value
+    finally:
+        dg.close()
+    solve_and_certify(mf, orc, ctx, g, algo, "bcsr", value, **best)

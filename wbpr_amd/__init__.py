@@ -98,8 +98,18 @@ class _Options(C.Structure):
         ("batch_algo", C.c_int32),
         ("batch_mode", C.c_int32),
         ("bfs_direction", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("class_lanes", C.c_uint16 * 5),
+        ("pad0", C.c_uint16),
+        ("reserved", C.c_int32 * 1),
     ]
+
+    def to_dict(self) -> dict:
+        """The tunable fields, in the keyword form options() accepts."""
+        return {"cycles_per_relabel": self.cycles_per_relabel, "tile_lanes": self.tile_lanes,
+                "bfs_block_frontier": self.bfs_block_frontier,
+                "relabel_time_ratio": round(float(self.relabel_time_ratio), 4),
+                "batch_algo": self.batch_algo, "batch_mode": self.batch_mode,
+                "bfs_direction": self.bfs_direction, "class_lanes": list(self.class_lanes)}
 
 
 class _Result(C.Structure):
@@ -205,6 +215,10 @@ _sig("mf_server_stats_get", C.c_int, _vp, C.POINTER(_ServerStats))
 _sig("mf_server_destroy", C.c_int, _vp)
 _sig("mf_footprint_bytes", C.c_int64, C.c_int32, C.c_int64)
 _sig("mf_flow_return_excess", C.c_int, C.POINTER(_Csr), _i32p)
+_sig("mf_autotune", C.c_int, _vp, _vp, C.c_int, C.POINTER(_Options), C.POINTER(_Options),
+     C.POINTER(C.c_double), C.POINTER(C.c_int32))
+_sig("mf_batch_autotune", C.c_int, _vp, C.POINTER(_Csr), C.c_int32, C.c_int32,
+     C.POINTER(_Options), C.POINTER(_Options), C.POINTER(C.c_double), C.POINTER(C.c_int32))
 
 #: every function the C header declares (checked by the export test)
 EXPORTED = [
@@ -217,6 +231,7 @@ EXPORTED = [
     "mf_graph_destroy", "mf_solve", "mf_rcsr_push_relabel", "mf_batch_solve",
     "mf_server_create", "mf_server_submit", "mf_server_progress", "mf_server_result",
     "mf_server_stats_get", "mf_server_destroy", "mf_footprint_bytes", "mf_flow_return_excess",
+    "mf_autotune", "mf_batch_autotune",
 ]
 
 
@@ -425,6 +440,9 @@ def options(**kw) -> _Options:
     for k, v in kw.items():
         if k == "relabel_time_ratio":
             o.relabel_time_ratio = float(v)
+        elif k == "class_lanes":
+            for i, x in enumerate(list(v)[:5]):
+                o.class_lanes[i] = int(x)
         else:
             setattr(o, k, int(v))
     return o
@@ -507,6 +525,21 @@ class Context:
         _check(_lib.mf_batch_solve(self._p, arr, k, C.byref(o), vals, C.byref(r)), "batch_solve")
         return np.array(vals[:], dtype=np.int64), Result._from(r)
 
+    def batch_autotune(self, graphs, sample: int = 256, **base):
+        """Tuned batch options (dict for batch_solve), best sample time in ms, trials run."""
+        k = len(graphs)
+        arr = (_Csr * k)()
+        keep = []
+        for i, g in enumerate(graphs):
+            c = g.csr()
+            keep.append(c)
+            arr[i] = c
+        o, best = options(**base), _Options()
+        ms, n = C.c_double(), C.c_int32()
+        _check(_lib.mf_batch_autotune(self._p, arr, k, sample, C.byref(o), C.byref(best),
+                                      C.byref(ms), C.byref(n)), "batch_autotune")
+        return best.to_dict(), ms.value, n.value
+
 
 class DeviceGraph:
     """Residual graph resident in HBM (mf_graph)."""
@@ -519,6 +552,14 @@ class DeviceGraph:
         r = _Result()
         _check(_lib.mf_graph_solve(self.ctx._p, self._p, algo, C.byref(o), C.byref(r)), "graph_solve")
         return Result._from(r)
+
+    def autotune(self, algo: int = ALGO_VC, **base):
+        """Tuned options (dict for solve), best solve time in ms, trial solves run."""
+        o, best = options(**base), _Options()
+        ms, n = C.c_double(), C.c_int32()
+        _check(_lib.mf_autotune(self.ctx._p, self._p, algo, C.byref(o), C.byref(best),
+                                C.byref(ms), C.byref(n)), "autotune")
+        return best.to_dict(), ms.value, n.value
 
     def read_flow(self, return_excess: bool = True) -> np.ndarray:
         fl = np.zeros(max(self.graph.m, 1), dtype=np.int32)

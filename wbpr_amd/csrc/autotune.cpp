@@ -1,0 +1,189 @@
+// Autotuner of the solver's launch parameters: trial solves over a small candidate set, by
+// coordinate descent (one parameter at a time, keeping the best so far).
+//
+//   tile widths       mf_options.class_lanes: lanes per vertex of each VC degree class.  Narrow
+//                     tiles put more vertices in flight per wave, wide tiles more slots per
+//                     vertex; the best split depends on the degree distribution.
+//   relabel cadence   mf_options.relabel_time_ratio: cycle time per global-relabel time.  Graphs
+//                     that strand excess want frequent relabels, long-path graphs fewer.
+//   BFS hand-over     mf_options.bfs_block_frontier: frontier size up to which all BFS levels run
+//                     inside one workgroup.
+//   batch algorithm   mf_options.batch_algo (batched solves only): TC or VC cycles.
+//
+// Lock-free schedules differ from run to run, so a candidate's time is the best of two solves.
+// Every trial must give the same max-flow value (the value is unique); a mismatch is reported as
+// MF_ERR_INTERNAL.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../../include/mf.h"
+
+namespace {
+
+constexpr int kRepeats = 2;
+
+// VC tile-width presets (lanes for degree classes <=8, <=16, <=32, <=1024, larger)
+const uint16_t kLanePresets[][5] = {
+    {8, 16, 32, 64, 256},  // default: one slot per lane per class bound, 4 in flight
+    {2, 4, 8, 32, 256},    // narrow: four slots in flight per lane cover each class bound
+    {4, 8, 16, 64, 256},   // middle
+    {1, 2, 8, 32, 256},    // low-degree vertices thread-centric
+};
+const float kRatios[] = {0.5f, 1.0f, 2.0f, 4.0f, 8.0f};
+const int32_t kFrontiers[] = {1024, 4096, 16384};
+
+float effective_ratio(const mf_options &o) {
+  return o.relabel_time_ratio > 0 ? o.relabel_time_ratio : 2.0f;
+}
+
+int32_t effective_frontier(const mf_options &o) {
+  return o.bfs_block_frontier > 0 ? o.bfs_block_frontier : 4096;
+}
+
+bool same_lanes(const uint16_t *a, const uint16_t *b) {
+  static const uint16_t def[5] = {8, 16, 32, 64, 256};
+  for (int k = 0; k < 5; ++k) {
+    const uint16_t x = a[k] ? a[k] : def[k], y = b[k] ? b[k] : def[k];
+    if (x != y) return false;
+  }
+  return true;
+}
+
+// Coordinate descent over the candidates; run(opts, ms) performs one timed trial.
+template <class Run>
+mf_status descend(mf_options cur, bool tune_lanes, bool tune_batch_algo, Run run,
+                  mf_options *best, double *best_ms) {
+  // the result names every tuned value explicitly (0 would mean "library default")
+  cur.relabel_time_ratio = effective_ratio(cur);
+  cur.bfs_block_frontier = effective_frontier(cur);
+  if (tune_lanes)
+    for (int k = 0; k < 5; ++k)
+      if (!cur.class_lanes[k]) cur.class_lanes[k] = kLanePresets[0][k];
+  double cur_ms = 0;
+  mf_status st = run(cur, cur_ms);
+  if (st != MF_OK) return st;
+  auto consider = [&](const mf_options &t) -> mf_status {
+    double ms = 0;
+    mf_status s = run(t, ms);
+    if (s != MF_OK) return s;
+    if (ms < cur_ms) {
+      cur = t;
+      cur_ms = ms;
+    }
+    return MF_OK;
+  };
+  if (tune_batch_algo) {
+    mf_options t = cur;
+    t.batch_algo = cur.batch_algo == 0 ? 1 : 0;
+    if ((st = consider(t)) != MF_OK) return st;
+  }
+  if (tune_lanes && (!tune_batch_algo || cur.batch_algo == 1)) {
+    for (const auto &p : kLanePresets) {
+      if (same_lanes(p, cur.class_lanes)) continue;
+      mf_options t = cur;
+      std::memcpy(t.class_lanes, p, sizeof(t.class_lanes));
+      if ((st = consider(t)) != MF_OK) return st;
+    }
+  }
+  const float r0 = effective_ratio(cur);
+  for (float r : kRatios) {
+    if (r == r0) continue;
+    mf_options t = cur;
+    t.relabel_time_ratio = r;
+    if ((st = consider(t)) != MF_OK) return st;
+  }
+  if (!tune_batch_algo) {  // the batched kernel runs its BFS inside each workgroup anyway
+    const int32_t f0 = effective_frontier(cur);
+    for (int32_t f : kFrontiers) {
+      if (f == f0) continue;
+      mf_options t = cur;
+      t.bfs_block_frontier = f;
+      if ((st = consider(t)) != MF_OK) return st;
+    }
+  }
+  *best = cur;
+  if (best_ms) *best_ms = cur_ms;
+  return MF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mf_status mf_autotune(mf_context *ctx, mf_graph *g, mf_algo algo, const mf_options *base,
+                      mf_options *best, double *best_ms, int32_t *trials) {
+  if (!ctx || !g || !best) return MF_ERR_INVALID_ARGUMENT;
+  if (algo != MF_ALGO_TC && algo != MF_ALGO_VC) return MF_ERR_INVALID_ARGUMENT;
+  mf_options o;
+  mf_options_default(&o);
+  if (base) o = *base;
+  o.profile_kernels = 0;
+  int64_t value = -1;
+  int32_t n = 0;
+  auto run = [&](const mf_options &t, double &ms) -> mf_status {
+    ms = 1e300;
+    for (int r = 0; r < kRepeats; ++r) {
+      mf_result res;
+      const mf_status st = mf_graph_solve(ctx, g, algo, &t, &res);
+      if (st != MF_OK) return st;
+      if (value >= 0 && res.value != value) return MF_ERR_INTERNAL;
+      value = res.value;
+      ms = std::min(ms, res.solve_ms);
+      ++n;
+    }
+    return MF_OK;
+  };
+  const mf_status st =
+      descend(o, algo == MF_ALGO_VC && o.tile_lanes == 0, false, run, best, best_ms);
+  if (trials) *trials = n;
+  return st;
+}
+
+mf_status mf_batch_autotune(mf_context *ctx, const mf_csr *inst, int32_t k, int32_t sample,
+                            const mf_options *base, mf_options *best, double *best_ms,
+                            int32_t *trials) {
+  if (!ctx || !inst || k <= 0 || !best) return MF_ERR_INVALID_ARGUMENT;
+  if (sample <= 0) sample = 256;
+  sample = std::min(sample, k);
+  // the largest instances set the tail of a persistent launch: a quarter of the sample is the
+  // largest ones, the rest an even spread over the others
+  std::vector<int32_t> order((size_t)k);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    return (int64_t)inst[a].n + inst[a].m > (int64_t)inst[b].n + inst[b].m;
+  });
+  const int32_t top = std::max(1, sample / 4);
+  std::vector<mf_csr> sub;
+  sub.reserve((size_t)sample);
+  for (int32_t i = 0; i < top; ++i) sub.push_back(inst[order[(size_t)i]]);
+  const int32_t rest = sample - top, pool = k - top;
+  for (int32_t i = 0; i < rest && pool > 0; ++i)
+    sub.push_back(inst[order[(size_t)top + (size_t)((int64_t)i * pool / rest)]]);
+  mf_options o;
+  mf_options_default(&o);
+  if (base) o = *base;
+  o.profile_kernels = 0;
+  std::vector<int64_t> ref, vals(sub.size());
+  int32_t n = 0;
+  auto run = [&](const mf_options &t, double &ms) -> mf_status {
+    ms = 1e300;
+    for (int r = 0; r < kRepeats; ++r) {
+      mf_result res;
+      const mf_status st =
+          mf_batch_solve(ctx, sub.data(), (int32_t)sub.size(), &t, vals.data(), &res);
+      if (st != MF_OK) return st;
+      if (!ref.empty() && ref != vals) return MF_ERR_INTERNAL;
+      ref = vals;
+      ms = std::min(ms, res.solve_ms);
+      ++n;
+    }
+    return MF_OK;
+  };
+  const mf_status st = descend(o, o.tile_lanes == 0, true, run, best, best_ms);
+  if (trials) *trials = n;
+  return st;
+}
+
+}  // extern "C"

@@ -15,7 +15,7 @@ struct ClassConfig {
   int maxdeg[5] = {0, 0, 0, 0, 0};
 };
 
-ClassConfig class_config(mf_algo algo, int tile_lanes);
+ClassConfig class_config(mf_algo algo, int tile_lanes, const uint16_t *class_lanes = nullptr);
 int class_of_degree(const ClassConfig &c, int64_t deg);
 
 int64_t algorithmic_bytes_cycle(mf_layout layout, int64_t active, int64_t slots, int64_t pushes,

@@ -296,6 +296,8 @@ __global__ __launch_bounds__(kBlock) void k_cycle(L g, CycleArgs a, Dispatch d) 
   Stats st{0, 0, 0, 0};
   switch (pick(d.lanes, c)) {
     case 1: cycle_class<1>(g, a, &S, c, cnt, b, nblk, st); break;
+    case 2: cycle_class<2>(g, a, &S, c, cnt, b, nblk, st); break;
+    case 4: cycle_class<4>(g, a, &S, c, cnt, b, nblk, st); break;
     case 8: cycle_class<8>(g, a, &S, c, cnt, b, nblk, st); break;
     case 16: cycle_class<16>(g, a, &S, c, cnt, b, nblk, st); break;
     case 32: cycle_class<32>(g, a, &S, c, cnt, b, nblk, st); break;
@@ -322,6 +324,8 @@ struct BfsArgs {
   int *flags;      // [0] a source was reached, [1] a worklist overflowed
   int *state;      // [0] level, [1] frontier size (exit state of the one-workgroup kernel)
   unsigned long long *stats;
+  int cls_mean[kNumClasses];  // mean residual degree of each frontier class
+  int wg_slots;               // the one-workgroup kernel hands over above this many frontier slots
 };
 
 // LDS stages of a grid-wide BFS launch (frontier and worklist appends), or none.
@@ -605,7 +609,7 @@ template <class L>
 __global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d, int level,
                                                      int fmax) {
   __shared__ int s_cnt[3][kNumClasses];
-  __shared__ int s_size;
+  __shared__ int s_size, s_big;
   long long nv = 0, ns = 0;
   const int tid = (int)threadIdx.x;
   if (tid < 3 * kNumClasses) {
@@ -616,13 +620,18 @@ __global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d,
   while (true) {
     if (tid == 0) {
       int sz = 0;
-      for (int c = 0; c < d.nclass; ++c) sz += s_cnt[level % 3][c];
+      long long slots = 0;  // estimated from the class means: hubs make a few vertices heavy
+      for (int c = 0; c < d.nclass; ++c) {
+        sz += s_cnt[level % 3][c];
+        slots += (long long)s_cnt[level % 3][c] * pick(a.cls_mean, c);
+      }
       s_size = sz;
+      s_big = slots > a.wg_slots;
     }
     if (tid < kNumClasses) s_cnt[(level + 2) % 3][tid] = 0;  // next level's output counters
     __syncthreads();
     const int size = s_size;
-    if (size == 0 || size > fmax) break;
+    if (size == 0 || size > fmax || s_big) break;
     ListSet in = (level & 1) ? a.fr[1] : a.fr[0];
     in.cnt = s_cnt[level % 3];
     in.cstride = 1;
@@ -747,6 +756,8 @@ __device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &c
       discharge_tile<TT>(g, ca, nullptr, ca.in.items[base + i], tid % TT, st);     \
     break;
     MF_BATCH_CASE(1)
+    MF_BATCH_CASE(2)
+    MF_BATCH_CASE(4)
     MF_BATCH_CASE(8)
     MF_BATCH_CASE(16)
     MF_BATCH_CASE(32)
@@ -801,6 +812,8 @@ __device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, i
   ba.flags = s_flag;
   ba.state = nullptr;
   ba.stats = nullptr;
+  for (int c = 0; c < kNumClasses; ++c) ba.cls_mean[c] = 0;  // (used by the grid-wide BFS only)
+  ba.wg_slots = INT_MAX;
   CycleArgs ca;
   ca.h = b.h;
   ca.e = b.e;
@@ -965,6 +978,7 @@ static double now_ms() {
 constexpr float kDefaultRelabelRatio = 2.0f;
 constexpr int kMaxAdaptiveCycles = 1 << 14;  // cap of cycles per round in adaptive mode
 constexpr int kDefaultBfsFrontier = 4096;    // one-workgroup BFS up to this frontier size
+constexpr int kWgBfsSlots = 32768;           // ... and up to about this many frontier slots
 constexpr int kCycleChunk = 64;              // most cycles launched between two worklist checks
 constexpr int kFirstChunk = 8;               // first chunk, before a cycle has been timed
 // Direction of the grid-wide BFS levels (Beamer's rule): bottom-up once the frontier's slots
@@ -983,7 +997,7 @@ static mf_status dmalloc(T **p, size_t count) {
   return MF_OK;
 }
 
-ClassConfig class_config(mf_algo algo, int tile_lanes) {
+ClassConfig class_config(mf_algo algo, int tile_lanes, const uint16_t *class_lanes) {
   ClassConfig c;
   if (algo == MF_ALGO_TC) {
     c.nclass = 1;
@@ -998,7 +1012,7 @@ ClassConfig class_config(mf_algo algo, int tile_lanes) {
     const int lanes[5] = {8, 16, 32, 64, kBlock};
     const int maxdeg[5] = {8, 16, 32, 1024, INT32_MAX};
     for (int k = 0; k < 5; ++k) {
-      c.lanes[k] = lanes[k];
+      c.lanes[k] = (class_lanes && class_lanes[k]) ? class_lanes[k] : lanes[k];
       c.maxdeg[k] = maxdeg[k];
     }
   }
@@ -1270,7 +1284,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
                            const mf_options &o, mf_result *res) {
   hipStream_t st = ctx->stream;
   const int n = G.n;
-  const ClassConfig cfg = class_config(algo, o.tile_lanes);
+  const ClassConfig cfg = class_config(algo, o.tile_lanes, o.class_lanes);
   int64_t pop[kNumClasses] = {0};
   if (cfg.nclass == 1) pop[0] = n;
   else
@@ -1325,6 +1339,9 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   ba.active.cnt = ccnt[0];
   ba.flags = G.ctl + CTL_FLAGS;
   ba.state = G.ctl + CTL_STATE;
+  for (int c = 0; c < kNumClasses; ++c)
+    ba.cls_mean[c] = G.vc_pop[c] ? (int)((G.class_slots[c] + G.vc_pop[c] - 1) / G.vc_pop[c]) : 0;
+  ba.wg_slots = kWgBfsSlots;
   ba.stats = G.stats;
 
   CycleArgs ca;
@@ -1385,6 +1402,12 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     k_bfs_seed<<<kgrid, 256, 0, st>>>(ba, G.src, G.snk, G.k);
     int level = 0, levels_total = 0;
     double explored = 0;  // slots of the frontiers expanded by grid-wide levels
+    auto frontier_slots = [&](int lv) {
+      double sl = 0;
+      for (int c = 0; c < bcfg.nclass; ++c)
+        sl += (double)hstate[ctl_cnt(CTL_FCNT, lv % 3, c)] * ba.cls_mean[c];
+      return sl;
+    };
     for (;;) {
       k_bfs_wg<L><<<1, kWgBlock, 0, st>>>(lay, ba, bdisp, level, fmax);
       HIP_TRY(hipGetLastError());
@@ -1395,13 +1418,9 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       if (size == 0) break;
       bool bottom_up = false;
       for (;;) {  // frontier too large for one workgroup: one launch per level
+        // frontier slots estimated from its class counts and the classes' mean degrees
+        const double mf = frontier_slots(level);
         if (bu_allowed) {
-          // frontier slots estimated from its class counts and the classes' mean degrees
-          double mf = 0;
-          for (int c = 0; c < bcfg.nclass; ++c) {
-            const int cnt = hstate[ctl_cnt(CTL_FCNT, level % 3, c)];
-            if (G.vc_pop[c]) mf += (double)cnt * (double)G.class_slots[c] / (double)G.vc_pop[c];
-          }
           const double mu = std::max(0.0, (double)G.S - explored);
           if (!bottom_up && mf * kBottomUpAlpha > mu) bottom_up = true;
           else if (bottom_up && (int64_t)size * kBottomUpBeta < G.nonzero) bottom_up = false;
@@ -1421,7 +1440,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
         HIP_TRY(hipStreamSynchronize(st));
         size = 0;
         for (int c = 0; c < bcfg.nclass; ++c) size += hstate[ctl_cnt(CTL_FCNT, level % 3, c)];
-        if (size <= fmax_low) break;
+        if (size <= fmax_low && frontier_slots(level) <= ba.wg_slots / 2) break;
       }
       if (size == 0) break;
     }
@@ -1560,9 +1579,12 @@ mf_status solve_graph(mf_context *ctx, DevGraph &G, mf_algo algo, const mf_optio
 }
 
 static mf_status check_options(const mf_options &o) {
-  if (o.tile_lanes != 0 && o.tile_lanes != 1 && o.tile_lanes != 8 && o.tile_lanes != 16 &&
-      o.tile_lanes != 32 && o.tile_lanes != 64 && o.tile_lanes != kBlock)
-    return MF_ERR_INVALID_ARGUMENT;
+  auto lanes_ok = [](int v) {
+    return v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == 32 || v == 64 || v == kBlock;
+  };
+  if (o.tile_lanes != 0 && !lanes_ok(o.tile_lanes)) return MF_ERR_INVALID_ARGUMENT;
+  for (int k = 0; k < kNumClasses; ++k)
+    if (o.class_lanes[k] != 0 && !lanes_ok(o.class_lanes[k])) return MF_ERR_INVALID_ARGUMENT;
   if (o.cycles_per_relabel < 0 || o.max_rounds < 0 || o.bfs_block_frontier < 0)
     return MF_ERR_INVALID_ARGUMENT;
   if (o.bfs_direction < 0 || o.bfs_direction > 2 || o.profile_kernels < 0)
@@ -1577,7 +1599,7 @@ static mf_status run_batch_wg(mf_context *ctx, DevGraph &G, const int32_t *vbase
   hipStream_t st = ctx->stream;
   const int32_t k = G.k;
   const ClassConfig vc = class_config(MF_ALGO_VC, 0);
-  const ClassConfig cc = class_config(algo, o.tile_lanes);
+  const ClassConfig cc = class_config(algo, o.tile_lanes, o.class_lanes);
   // slot degree and class of every vertex (the same classes upload() stored on the device)
   std::vector<int32_t> indeg((size_t)G.n, 0);
   for (int64_t i = 0; i < G.m; ++i) indeg[(size_t)G.h_dst[(size_t)i]]++;
