@@ -103,7 +103,11 @@ typedef struct mf_options {
                                  <=8, <=16, <=32, <=1024, larger slots; 0 = default (8, 16, 32, 64,
                                  256); allowed 1, 2, 4, 8, 16, 32, 64, 256 (256 = whole block) */
   uint16_t pad0;
-  int32_t reserved[1];
+  int32_t forward_hops;       /* push forwarding: a tile whose push activates a vertex may discharge
+                                 it in the same cycle, up to this many vertices per cycle;
+                                 0 = default (2), -1 = off (every activated vertex waits for the
+                                 next cycle) */
+  int32_t reserved[4];
 } mf_options;
 
 typedef struct mf_result {

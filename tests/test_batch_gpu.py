@@ -54,6 +54,15 @@ def test_batch_mixed_generators_modes(mf, ctx, golden, mode):
         assert values.tolist() == [v for _, v in inst], (mode, algo)
 
 
+@pytest.mark.parametrize("hops", [-1, 32])
+def test_batch_forwarding(mf, ctx, golden, hops):
+    rng = np.random.default_rng(9)
+    inst = mixed_instances(mf, golden, rng, 24)
+    for algo in (0, 1):
+        values, _ = ctx.batch_solve([g for g, _ in inst], batch_algo=algo, forward_hops=hops)
+        assert values.tolist() == [v for _, v in inst], (algo, hops)
+
+
 def test_batch_fixed_cadence_and_limits(mf, ctx, golden):
     g, v = generated_graph(mf, golden, "genrmf12_s2")
     values, _ = ctx.batch_solve([g, g, g], cycles_per_relabel=8)

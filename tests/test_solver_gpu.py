@@ -92,7 +92,8 @@ def test_launch_geometry_stress(mf, orc, ctx, golden):
             opts = dict(cycles_per_relabel=int(rng.choice([1, 3, 16, 64, 1024])),
                         tile_lanes=int(rng.choice([0, 1, 8, 16, 32, 64, 256])),
                         bfs_block_frontier=int(rng.choice([1, 8, 64, 4096])),
-                        bfs_direction=int(rng.choice([0, 1, 2])))
+                        bfs_direction=int(rng.choice([0, 1, 2])),
+                        forward_hops=int(rng.choice([-1, 0, 1, 32])))
             r = dg.solve(mf.ALGO_VC, **opts)
             assert r.value == value, opts
     finally:
@@ -277,3 +278,12 @@ def test_autotune(mf, orc, ctx, golden, algo):
     finally:
         dg.close()
     solve_and_certify(mf, orc, ctx, g, algo, "bcsr", value, **best)
+
+
+@pytest.mark.parametrize("hops", [-1, 1, 64])
+@pytest.mark.parametrize("name", ["rlg32_s7", "genrmf16_s1", "rmat12_s3", "bip_3000_2000_s9"])
+def test_push_forwarding(mf, orc, ctx, golden, name, hops):
+    # a tile that activates a vertex discharges it in the same cycle (-1: never)
+    g, value = generated_graph(mf, golden, name)
+    for algo in ALGOS:
+        solve_and_certify(mf, orc, ctx, g, algo, "bcsr", value, forward_hops=hops)

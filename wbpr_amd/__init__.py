@@ -100,7 +100,8 @@ class _Options(C.Structure):
         ("bfs_direction", C.c_int32),
         ("class_lanes", C.c_uint16 * 5),
         ("pad0", C.c_uint16),
-        ("reserved", C.c_int32 * 1),
+        ("forward_hops", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
     def to_dict(self) -> dict:
@@ -109,7 +110,8 @@ class _Options(C.Structure):
                 "bfs_block_frontier": self.bfs_block_frontier,
                 "relabel_time_ratio": round(float(self.relabel_time_ratio), 4),
                 "batch_algo": self.batch_algo, "batch_mode": self.batch_mode,
-                "bfs_direction": self.bfs_direction, "class_lanes": list(self.class_lanes)}
+                "bfs_direction": self.bfs_direction, "class_lanes": list(self.class_lanes),
+                "forward_hops": self.forward_hops}
 
 
 class _Result(C.Structure):

@@ -8,6 +8,8 @@
 //                     that strand excess want frequent relabels, long-path graphs fewer.
 //   BFS hand-over     mf_options.bfs_block_frontier: frontier size up to which all BFS levels run
 //                     inside one workgroup.
+//   forwarding        mf_options.forward_hops: vertices a tile discharges per cycle by following
+//                     its own pushes (long augmenting paths want many, churning graphs few).
 //   batch algorithm   mf_options.batch_algo (batched solves only): TC or VC cycles.
 //
 // Lock-free schedules differ from run to run, so a candidate's time is the best of two solves.
@@ -32,6 +34,7 @@ const uint16_t kLanePresets[][5] = {
     {1, 2, 8, 32, 256},    // low-degree vertices thread-centric
 };
 const float kRatios[] = {0.5f, 1.0f, 2.0f, 4.0f, 8.0f};
+const int32_t kForwardHops[] = {-1, 2, 8, 32};
 const int32_t kFrontiers[] = {1024, 4096, 16384};
 
 float effective_ratio(const mf_options &o) {
@@ -58,6 +61,7 @@ mf_status descend(mf_options cur, bool tune_lanes, bool tune_batch_algo, Run run
   // the result names every tuned value explicitly (0 would mean "library default")
   cur.relabel_time_ratio = effective_ratio(cur);
   cur.bfs_block_frontier = effective_frontier(cur);
+  if (cur.forward_hops == 0) cur.forward_hops = 2;
   if (tune_lanes)
     for (int k = 0; k < 5; ++k)
       if (!cur.class_lanes[k]) cur.class_lanes[k] = kLanePresets[0][k];
@@ -86,6 +90,13 @@ mf_status descend(mf_options cur, bool tune_lanes, bool tune_batch_algo, Run run
       std::memcpy(t.class_lanes, p, sizeof(t.class_lanes));
       if ((st = consider(t)) != MF_OK) return st;
     }
+  }
+  const int32_t h0 = cur.forward_hops;
+  for (int32_t h : kForwardHops) {
+    if (h == h0) continue;
+    mf_options t = cur;
+    t.forward_hops = h;
+    if ((st = consider(t)) != MF_OK) return st;
   }
   const float r0 = effective_ratio(cur);
   for (float r : kRatios) {

@@ -300,6 +300,13 @@ __device__ __forceinline__ unsigned long long tile_min(unsigned long long v) {
 }
 
 template <int T>
+__device__ __forceinline__ int tile_max(int v) {
+#pragma unroll
+  for (int off = T / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, T));
+  return v;
+}
+
+template <int T>
 __device__ __forceinline__ long long tile_sum(long long v) {
 #pragma unroll
   for (int off = T / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, T);

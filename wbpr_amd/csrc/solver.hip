@@ -51,6 +51,9 @@ struct CycleArgs {
   ListSet in, out;
   int *clear_cnt;
   unsigned long long *stats;
+  int fwd_hops;  // push forwarding: further vertices a tile may discharge in the same cycle
+  int *stamp;    // per vertex: the last cycle that discharged it
+  int cycle;     // this cycle's number (increasing within a solve)
 };
 
 // Per-lane scan state: the lane's lowest (height, key) over its residual slots and, for that
@@ -131,10 +134,16 @@ struct Plan {
 // remainder (or it relabelled below H); a pusher adds w iff its atomicAdd found e(w) == 0.
 // Whichever of two atomics on one excess comes second sees the other's effect, so exactly one
 // of them appends.
+//
+// Push forwarding: the pusher that found e(w) == 0 holds w's only ticket into the next
+// worklist, and no one else is discharging w (a vertex queued for this cycle keeps positive
+// excess until its own step).  Instead of queueing w it may discharge w itself, now: the
+// returned w (or -1) is what the caller continues with.  fcls: forwarding allowed into vertices
+// of degree class <= fcls (-1: no forwarding), so a hub never lands on a narrow tile.
 template <class L>
-__device__ __forceinline__ void finish_vertex(const L &g, const CycleArgs &a, Stage *S, int u,
-                                              int hu, const Plan &p, const Cand &mine,
-                                              bool leader, int deg, Stats &st) {
+__device__ __forceinline__ int finish_vertex(const L &g, const CycleArgs &a, Stage *S, int u,
+                                             int hu, const Plan &p, const Cand &mine,
+                                             bool leader, int deg, int fcls, Stats &st) {
   bool want_u = false, want_w = false;
   if (leader) {
     st.active++;
@@ -162,6 +171,15 @@ __device__ __forceinline__ void finish_vertex(const L &g, const CycleArgs &a, St
       want_u = old_u - p.total > 0;
     }
   }
+  // forward only into a vertex not yet discharged in this cycle: a second discharge in one
+  // launch could read residuals and excess from cache lines older than the first one's atomics
+  // (too high), where a first discharge can only see values that are too low
+  int fwd = -1;
+  if (want_w && fcls >= 0 && (!a.vcls || a.vcls[mine.w] <= fcls) &&
+      atomicExch(&a.stamp[mine.w], a.cycle) != a.cycle) {
+    fwd = mine.w;
+    want_w = false;
+  }
   const int cu = a.vcls ? a.vcls[u] : 0;
   const int cw = (want_w && a.vcls) ? a.vcls[mine.w] : 0;
   if (S) {
@@ -171,6 +189,7 @@ __device__ __forceinline__ void finish_vertex(const L &g, const CycleArgs &a, St
     append(a.out, u, want_u, cu, a.nclass);
     append(a.out, mine.w, want_w, cw, a.nclass);
   }
+  return fwd;
 }
 
 __device__ __forceinline__ void plan_heights(unsigned long long best, int hu, int H, Plan &p) {
@@ -184,13 +203,15 @@ __device__ __forceinline__ long long admissible_residual(const Plan &p, const Ca
 }
 
 // One discharge step of u by a tile of T lanes (T <= 64, inside one wave).
+// Returns the vertex the tile continues with (push forwarding, see finish_vertex) or -1; fcls is
+// the forwarding class limit (-1: none).
 template <int T, class L>
-__device__ __forceinline__ void discharge_tile(const L &g, const CycleArgs &a, Stage *S, int u,
-                                               int lane, Stats &st) {
+__device__ __forceinline__ int discharge_tile(const L &g, const CycleArgs &a, Stage *S, int u,
+                                              int lane, int fcls, Stats &st) {
   const int hu = a.h[u];
   // e(u) grows while the tile runs: every lane uses lane 0's read (the value subtracted below)
   const int eu = __shfl(a.e[u], 0, T);
-  if (hu >= a.H) return;  // became inactive after it was queued (all lanes read the same value)
+  if (hu >= a.H) return -1;  // became inactive after it was queued (all lanes read the same value)
   Cand c;
   int deg;
   scan_slots<scan_batch<T>()>(g, a.h, u, lane, T, c, deg);
@@ -203,6 +224,7 @@ __device__ __forceinline__ void discharge_tile(const L &g, const CycleArgs &a, S
     const long long sum = __shfl(incl, T - 1, T);
     p.d = (int)max(0LL, min(want, (long long)eu - (incl - want)));
     p.total = (int)min((long long)eu, sum);
+    fcls = -1;  // several lanes push: no single vertex to continue with
   } else {
     // low-degree vertices push along the minimum arc only: fanning small excesses out to many
     // neighbours (unit-capacity bipartite networks) multiplies the pushes that follow
@@ -211,7 +233,12 @@ __device__ __forceinline__ void discharge_tile(const L &g, const CycleArgs &a, S
     p.d = winner ? total : 0;
     p.total = total;
   }
-  finish_vertex(g, a, S, u, hu, p, c, lane == 0, deg, st);
+  const int f = finish_vertex(g, a, S, u, hu, p, c, lane == 0, deg, fcls, st);
+  if constexpr (T == 1) {
+    return f;
+  } else {
+    return (int)tile_max<T>(f);  // only the winner lane can hold a vertex
+  }
 }
 
 // One discharge step of a hub vertex by the whole block (kBlock lanes, LDS reductions).
@@ -252,7 +279,17 @@ __device__ __forceinline__ void discharge_block(const L &g, const CycleArgs &a, 
   incl += before;
   p.d = (int)max(0LL, min(want, (long long)eu - (incl - want)));
   p.total = (int)min((long long)eu, sum);
-  finish_vertex(g, a, S, u, hu, p, c, threadIdx.x == 0, deg, st);
+  finish_vertex(g, a, S, u, hu, p, c, threadIdx.x == 0, deg, -1, st);
+}
+
+// The tile that takes a queued vertex stamps it with the cycle before its step, so no forwarder
+// discharges it a second time in the same cycle (finish_vertex).  The stamp's old value is
+// consumed, which orders the exchange before the step's own atomics.
+template <int T>
+__device__ __forceinline__ bool claim_queued(const CycleArgs &a, int u, int lane) {
+  int old = 0;
+  if (lane == 0) old = atomicExch(&a.stamp[u], a.cycle);
+  return __shfl(old, 0, T) != a.cycle;
 }
 
 template <int T, class L>
@@ -264,8 +301,12 @@ __device__ __forceinline__ void cycle_class(const L &g, const CycleArgs &a, Stag
     const int tile = b * tpb + (int)threadIdx.x / T;
     const int ntiles = nblk * tpb;
     const int lane = (int)threadIdx.x % T;
-    for (int i = tile; i < cnt; i += ntiles)
-      discharge_tile<T>(g, a, S, a.in.items[base + i], lane, st);
+    for (int i = tile; i < cnt; i += ntiles) {
+      int v = a.in.items[base + i];
+      if (a.fwd_hops > 0 && !claim_queued<T>(a, v, lane)) continue;
+      for (int hop = 0; v >= 0; ++hop)
+        v = discharge_tile<T>(g, a, S, v, lane, hop < a.fwd_hops ? c : -1, st);
+    }
   } else {
     for (int i = b; i < cnt; i += nblk) discharge_block(g, a, S, a.in.items[base + i], st);
   }
@@ -668,11 +709,12 @@ __global__ void k_reset_pairs(int2 *pairs, const int *cap0, long long count, int
   }
 }
 
-__global__ void k_reset_vertices(int *h, int *e, int n, int *bcf, long long m) {
+__global__ void k_reset_vertices(int *h, int *e, int *stamp, int n, int *bcf, long long m) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < n; v += stride) {
     h[v] = 0;
     e[v] = 0;
+    stamp[v] = -1;
   }
   if (bcf)
     for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < m; j += stride)
@@ -742,6 +784,8 @@ struct BatchArgs {
   float ratio;                     // cycle time / relabel time per round
   int max_rounds;
   int max_cycles;                  // per round
+  int fwd_hops;                    // push forwarding hops per cycle
+  int *stamp;                      // per vertex: last cycle that discharged it
 };
 
 template <class L>
@@ -752,8 +796,12 @@ __device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &c
   switch (lanes) {
 #define MF_BATCH_CASE(TT)                                                          \
   case TT:                                                                         \
-    for (int i = tid / TT; i < cnt; i += kBlock / TT)                               \
-      discharge_tile<TT>(g, ca, nullptr, ca.in.items[base + i], tid % TT, st);     \
+    for (int i = tid / TT; i < cnt; i += kBlock / TT) {                             \
+      int v = ca.in.items[base + i];                                               \
+      if (ca.fwd_hops > 0 && !claim_queued<TT>(ca, v, tid % TT)) continue;         \
+      for (int hop = 0; v >= 0; ++hop)                                             \
+        v = discharge_tile<TT>(g, ca, nullptr, v, tid % TT, hop < ca.fwd_hops ? c : -1, st); \
+    }                                                                              \
     break;
     MF_BATCH_CASE(1)
     MF_BATCH_CASE(2)
@@ -822,6 +870,9 @@ __device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, i
   ca.nclass = b.nclass;
   ca.clear_cnt = nullptr;
   ca.stats = nullptr;
+  ca.fwd_hops = b.fwd_hops;
+  ca.stamp = b.stamp;
+  ca.cycle = 0;
 
   // preflow: saturate the source's residual arcs
 #pragma unroll
@@ -894,6 +945,7 @@ __device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, i
     // ---- push-relabel cycles ----
     const long long t_cyc = wall_clock64();
     for (int q = 0; q < b.max_cycles; ++q) {
+      ++ca.cycle;  // increasing over the whole solve (stamps of earlier rounds never match)
       ca.in = (q & 1) ? S1 : S0;
       ca.in.cnt = s_c[q % 3];
       ca.out = (q & 1) ? S0 : S1;
@@ -979,6 +1031,7 @@ constexpr float kDefaultRelabelRatio = 2.0f;
 constexpr int kMaxAdaptiveCycles = 1 << 14;  // cap of cycles per round in adaptive mode
 constexpr int kDefaultBfsFrontier = 4096;    // one-workgroup BFS up to this frontier size
 constexpr int kWgBfsSlots = 32768;           // ... and up to about this many frontier slots
+constexpr int kDefaultForwardHops = 2;       // push forwarding: vertices a tile chains per cycle
 constexpr int kCycleChunk = 64;              // most cycles launched between two worklist checks
 constexpr int kFirstChunk = 8;               // first chunk, before a cycle has been timed
 // Direction of the grid-wide BFS levels (Beamer's rule): bottom-up once the frontier's slots
@@ -1105,6 +1158,7 @@ struct DevGraph {
   long long *values = nullptr;
   // state
   int *h = nullptr, *e = nullptr;
+  int *stamp = nullptr;  // last cycle that discharged each vertex (push forwarding)
   int *lists = nullptr;  // 4 sets x n items
   int *ctl = nullptr;    // counters and flags
   unsigned long long *stats = nullptr;
@@ -1140,7 +1194,7 @@ constexpr int ctl_cnt(int set, int q, int c) { return set + (q * kNumClasses + c
 static void free_dev(DevGraph *g) {
   void *ptrs[] = {g->off, g->dc,  g->rev, g->dst,    g->cr,    g->fdc,   g->bcf,
                   g->roff, g->rdi, g->cap0, g->src,   g->snk,   g->values, g->h,
-                  g->e,   g->lists, g->ctl, g->stats, g->vcls, g->vorder};
+                  g->e,   g->lists, g->ctl, g->stats, g->vcls, g->vorder, g->stamp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -1265,6 +1319,7 @@ static mf_status upload(mf_context *ctx, const mf_csr *c, const int32_t *src, co
     for (int32_t v = 0; v < c->n; ++v) degree[(size_t)v] = B.off[(size_t)v + 1] - B.off[(size_t)v];
   }
   if ((rc = dmalloc(&G.h, (size_t)c->n)) || (rc = dmalloc(&G.e, (size_t)c->n)) ||
+      (rc = dmalloc(&G.stamp, (size_t)c->n)) ||
       (rc = dmalloc(&G.lists, (size_t)4 * c->n)) || (rc = dmalloc(&G.ctl, (size_t)CTL_SIZE)) ||
       (rc = dmalloc(&G.stats, (size_t)kStatSlot * kStatSpread)) || (rc = dmalloc(&G.src, (size_t)k)) ||
       (rc = dmalloc(&G.snk, (size_t)k)) || (rc = dmalloc(&G.values, (size_t)k)))
@@ -1351,6 +1406,8 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   ca.vcls = vcls;
   ca.nclass = cfg.nclass;
   ca.stats = G.stats;
+  ca.fwd_hops = o.forward_hops < 0 ? 0 : (o.forward_hops > 0 ? o.forward_hops : kDefaultForwardHops);
+  ca.stamp = G.stamp;
 
   const int fmax = o.bfs_block_frontier > 0 ? o.bfs_block_frontier : kDefaultBfsFrontier;
   const int fmax_low = std::max(1, fmax / 2);
@@ -1381,7 +1438,8 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   if (G.layout == MF_LAYOUT_BCSR) k_reset_pairs<<<rgrid, 256, 0, st>>>(G.dc, G.cap0, G.S, 1);
   else if (G.layout == MF_LAYOUT_BCSR2) k_reset_pairs<<<rgrid, 256, 0, st>>>(G.cr, G.cap0, G.S, 0);
   else k_reset_pairs<<<rgrid, 256, 0, st>>>(G.fdc, G.cap0, G.m, 1);
-  k_reset_vertices<<<rgrid, 256, 0, st>>>(G.h, G.e, n, G.layout == MF_LAYOUT_RCSR ? G.bcf : nullptr,
+  k_reset_vertices<<<rgrid, 256, 0, st>>>(G.h, G.e, G.stamp, n,
+                                          G.layout == MF_LAYOUT_RCSR ? G.bcf : nullptr,
                                           G.m);
   HIP_TRY(hipMemsetAsync(G.stats, 0, sizeof(unsigned long long) * kStatSlot * kStatSpread, st));
   HIP_TRY(hipMemsetAsync(G.ctl, 0, sizeof(int) * CTL_SIZE, st));
@@ -1501,6 +1559,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
         ca.out = sets[2 + ((q + 1) & 1)];
         ca.out.cnt = ccnt[(q + 1) % 3];
         ca.clear_cnt = ccnt[(q + 2) % 3];
+        ca.cycle = (int)(cycles + q);
         if (o.profile_kernels > 0 && q % o.profile_kernels == 0) {
           // every profile_kernels-th launch carries start/end events in its dispatch packet
           const size_t pe = (size_t)half * 2 * kCycleChunk + (size_t)2 * timed++;
@@ -1587,7 +1646,8 @@ static mf_status check_options(const mf_options &o) {
     if (o.class_lanes[k] != 0 && !lanes_ok(o.class_lanes[k])) return MF_ERR_INVALID_ARGUMENT;
   if (o.cycles_per_relabel < 0 || o.max_rounds < 0 || o.bfs_block_frontier < 0)
     return MF_ERR_INVALID_ARGUMENT;
-  if (o.bfs_direction < 0 || o.bfs_direction > 2 || o.profile_kernels < 0)
+  if (o.bfs_direction < 0 || o.bfs_direction > 2 || o.profile_kernels < 0 ||
+      o.forward_hops < -1)
     return MF_ERR_INVALID_ARGUMENT;
   return MF_OK;
 }
@@ -1671,12 +1731,14 @@ static mf_status run_batch_wg(mf_context *ctx, DevGraph &G, const int32_t *vbase
   b.ratio = o.relabel_time_ratio > 0 ? o.relabel_time_ratio : kDefaultRelabelRatio;
   b.max_rounds = o.max_rounds > 0 ? (int)std::min<int64_t>(o.max_rounds, INT32_MAX) : 1000000;
   b.max_cycles = o.cycles_per_relabel > 0 ? o.cycles_per_relabel : kMaxAdaptiveCycles;
+  b.fwd_hops = o.forward_hops < 0 ? 0 : (o.forward_hops > 0 ? o.forward_hops : kDefaultForwardHops);
+  b.stamp = G.stamp;
   if (o.cycles_per_relabel > 0) b.ratio = 1e30f;  // fixed cadence: only the cycle cap ends a round
   const int nwg = std::max(1, std::min(k, kBatchGroupsPerCU * ctx->num_cu));
   const double t0 = now_ms();
   const int rgrid = std::max(1, (int)std::min<int64_t>((G.S + 255) / 256, 16 * ctx->num_cu));
   k_reset_pairs<<<rgrid, 256, 0, st>>>(G.dc, G.cap0, G.S, 1);
-  k_reset_vertices<<<rgrid, 256, 0, st>>>(G.h, G.e, G.n, nullptr, 0);
+  k_reset_vertices<<<rgrid, 256, 0, st>>>(G.h, G.e, G.stamp, G.n, nullptr, 0);
   if (hipMemsetAsync(G.stats, 0, sizeof(unsigned long long) * kStatSlot * kStatSpread, st) != hipSuccess ||
       hipMemsetAsync(G.ctl, 0, sizeof(int) * CTL_SIZE, st) != hipSuccess ||
       hipMemsetAsync(d_next, 0, sizeof(int), st) != hipSuccess) {
@@ -1967,6 +2029,7 @@ mf_status mf_rcsr_push_relabel(mf_context *ctx, int algo_type, int V, int E, int
       (st = dmalloc(&G.bcf, (size_t)E)) || (st = dmalloc(&G.roff, (size_t)V + 1)) ||
       (st = dmalloc(&G.rdi, (size_t)E)) || (st = dmalloc(&G.cap0, (size_t)E)) ||
       (st = dmalloc(&G.h, (size_t)V)) || (st = dmalloc(&G.e, (size_t)V)) ||
+      (st = dmalloc(&G.stamp, (size_t)V)) ||
       (st = dmalloc(&G.lists, (size_t)4 * V)) || (st = dmalloc(&G.ctl, (size_t)CTL_SIZE)) ||
       (st = dmalloc(&G.stats, (size_t)kStatSlot * kStatSpread)) || (st = dmalloc(&G.src, (size_t)1)) ||
       (st = dmalloc(&G.snk, (size_t)1)) || (st = dmalloc(&G.values, (size_t)1)))
