@@ -137,3 +137,33 @@ def test_batch_autotune(mf, orc, ctx):
     assert trials >= 8 and ms > 0 and best["batch_algo"] in (0, 1)
     values, _ = ctx.batch_solve(graphs, **best)
     assert values.tolist() == expect
+
+
+def test_server_large_requests_run_alone(mf, orc, ctx, golden):
+    # requests above 16,384 vertices run alone on the whole grid, the others in batched
+    # launches; every value is the instance's own
+    big = [mf.gen_bipartite(20000, 15000, 8.0, seed=77), mf.gen_rmat(16, 16, seed=4)]
+    expect_big = [orc.dinic_maxflow(g) for g in big]
+    rng = np.random.default_rng(31)
+    small = mixed_instances(mf, golden, rng, 12)
+    srv = mf.Server(ctx, hbm_budget_bytes=1 << 30, max_inflight_launches=3)
+    try:
+        ids = []
+        for i, (g, v) in enumerate(small):
+            ids.append((srv.submit(g), v))
+            if i in (3, 8):
+                j = (i == 8)
+                ids.append((srv.submit(big[j]), expect_big[j]))
+        pending = dict(ids)
+        t0 = time.time()
+        while pending and time.time() - t0 < 120:
+            srv.progress(wait=True)
+            for rid in list(pending):
+                r = srv.result(rid)
+                if r is not None:
+                    assert r[0] == pending.pop(rid)
+        assert not pending
+        st = srv.stats()
+        assert st["completed"] == len(ids) and st["launches"] >= 3
+    finally:
+        srv.close()

@@ -1,7 +1,9 @@
-// Serving loop: requests are admitted by device-memory footprint into batches that run
-// concurrently, each batch on its own context (HIP stream) driven by a worker thread, so work
-// of requests that arrive while others are on the GPU overlaps on the device.  Finished batches
-// are retired and their memory budget is reused.
+// Serving loop: requests are admitted by device-memory footprint into launches that run
+// concurrently, each on its own context (HIP stream) driven by a worker thread, so work of
+// requests that arrive while others are on the GPU overlaps on the device.  Small requests are
+// grouped into one persistent batched launch (a workgroup per instance); a request too large for
+// one workgroup to solve quickly runs alone on the whole grid.  Admission is first come, first
+// served; finished launches are retired and their memory budget is reused.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -31,6 +33,7 @@ struct Request {
   int32_t n, s, t;
   int64_t m;
   int64_t footprint;
+  bool solo = false;  // solved alone by the grid-wide solver
   int state = 0;  // 0 pending, 1 running, 2 done, 3 failed
   int64_t value = 0;
   double device_ms = 0;
@@ -55,6 +58,11 @@ struct Launch {
   bool finished = false;
   mf_status status = MF_OK;
 };
+
+// Requests above either size run alone on the whole grid: one workgroup would take tens of
+// milliseconds on them while the grid-wide solver takes a few.
+constexpr int32_t kSoloVertices = 16384;
+constexpr int64_t kSoloArcs = 262144;
 
 }  // namespace
 
@@ -93,9 +101,15 @@ static void run_launch(mf_server *srv, Launch *L) {
   if (stt == MF_OK) {
     mf_options o;
     mf_options_default(&o);
+    const bool solo = L->reqs.size() == 1 && L->reqs[0]->solo;
+    // batched: thread-centric cycles inside each workgroup, relabel every cycle-time ~ BFS time;
+    // solo: the single-instance grid-wide solver, vertex-centric tiles (any degree mix)
+    o.batch_mode = solo ? 1 : 0;
+    o.relabel_time_ratio = solo ? 0.0f : 1.0f;
     stt = batch_solve_packed(srv->ctxs[(size_t)L->ctx_slot], &packed, src.data(), snk.data(),
-                             vbase.data(), (int32_t)views.size(), MF_ALGO_TC, MF_LAYOUT_BCSR, o,
-                             values.data(), &res);
+                             vbase.data(), (int32_t)views.size(),
+                             solo ? MF_ALGO_VC : MF_ALGO_TC, MF_LAYOUT_BCSR, o, values.data(),
+                             &res);
   }
   std::lock_guard<std::mutex> lk(srv->mu);
   for (size_t i = 0; i < L->reqs.size(); ++i) {
@@ -138,13 +152,16 @@ static void admit(mf_server *srv) {
     if (slot < 0) break;
     std::unique_ptr<Launch> L(new Launch());
     int64_t free_bytes = srv->budget - srv->in_use;
+    // in arrival order: a large request alone, or a run of small ones up to the next large one
     while (!srv->pending.empty() && srv->pending.front()->footprint <= free_bytes) {
       auto r = srv->pending.front();
+      if (r->solo && !L->reqs.empty()) break;
       srv->pending.pop_front();
       free_bytes -= r->footprint;
       L->bytes += r->footprint;
       r->state = 1;
       L->reqs.push_back(r);
+      if (r->solo) break;
     }
     if (L->reqs.empty()) break;
     srv->in_use += L->bytes;
@@ -201,6 +218,7 @@ mf_status mf_server_submit(mf_server *srv, const mf_csr *g, int64_t *request_id)
   r->cap.resize((size_t)g->m);
   for (int64_t i = 0; i < g->m; ++i) r->cap[(size_t)i] = g->cap ? g->cap[i] : 1;
   r->footprint = mf_footprint_bytes(g->n, g->m);
+  r->solo = g->n > kSoloVertices || g->m > kSoloArcs;
   std::lock_guard<std::mutex> lk(srv->mu);
   srv->st.submitted++;
   if (r->footprint > srv->budget) {
