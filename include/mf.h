@@ -208,6 +208,10 @@ mf_status mf_graph_read_flow(mf_context *ctx, mf_graph *g, int32_t *flow_out, in
 mf_status mf_flow_return_excess(const mf_csr *g, int32_t *flow);
 /* Min-cut side after a solve: side[v] = 1 if v can reach t in the residual graph, else 0. */
 mf_status mf_graph_read_cut(mf_context *ctx, mf_graph *g, uint8_t *sink_side);
+/* The resident residual layout, downloaded in the mf_build_layout format (free with
+ * mf_layout_free).  mf_graph_upload builds it on the device; the two builders give identical
+ * arrays. */
+mf_status mf_graph_read_layout(mf_context *ctx, mf_graph *g, mf_layout_arrays *out);
 /* Heights and excesses after a solve (n entries each; either may be NULL). */
 mf_status mf_graph_read_state(mf_context *ctx, mf_graph *g, int32_t *height, int32_t *excess);
 mf_status mf_graph_destroy(mf_context *ctx, mf_graph *g);

@@ -287,3 +287,26 @@ def test_push_forwarding(mf, orc, ctx, golden, name, hops):
     g, value = generated_graph(mf, golden, name)
     for algo in ALGOS:
         solve_and_certify(mf, orc, ctx, g, algo, "bcsr", value, forward_hops=hops)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("name", ["snap_8.txt", "m50.max", "rlg16_s3", "rmat12_s3",
+                                  "bip_500_700_s5", "multigraph"])
+def test_device_layout_matches_host_builder(mf, ctx, golden, name, layout):
+    # the residual layout built on the device is array-for-array the host builder's
+    if name == "multigraph":  # parallel and antiparallel arcs, self-loops, isolated vertices
+        rng = np.random.default_rng(3)
+        g = random_graph(mf, rng, 300, 2500, cap_max=50)
+    elif name.endswith((".max", ".txt")):
+        g, _ = fixture_graph(mf, golden, name)
+    else:
+        g, _ = generated_graph(mf, golden, name)
+    host = mf.build_layout(g, mf.LAYOUTS[layout])
+    dg = ctx.upload(g, mf.LAYOUTS[layout])
+    try:
+        dev = dg.read_layout()
+    finally:
+        dg.close()
+    assert host.keys() == dev.keys()
+    for k in host:
+        assert np.array_equal(host[k], dev[k]), k

@@ -201,6 +201,7 @@ _sig("mf_graph_solve", C.c_int, _vp, _vp, C.c_int, C.POINTER(_Options), C.POINTE
 _sig("mf_graph_read_flow", C.c_int, _vp, _vp, _i32p, C.c_int32)
 _sig("mf_graph_read_cut", C.c_int, _vp, _vp, C.POINTER(C.c_uint8))
 _sig("mf_graph_read_state", C.c_int, _vp, _vp, _i32p, _i32p)
+_sig("mf_graph_read_layout", C.c_int, _vp, _vp, C.POINTER(_LayoutArrays))
 _sig("mf_graph_destroy", C.c_int, _vp, _vp)
 _sig("mf_solve", C.c_int, _vp, C.POINTER(_Csr), C.c_int, C.c_int, C.POINTER(_Options),
      C.POINTER(_Result), _i32p)
@@ -233,7 +234,7 @@ EXPORTED = [
     "mf_graph_destroy", "mf_solve", "mf_rcsr_push_relabel", "mf_batch_solve",
     "mf_server_create", "mf_server_submit", "mf_server_progress", "mf_server_result",
     "mf_server_stats_get", "mf_server_destroy", "mf_footprint_bytes", "mf_flow_return_excess",
-    "mf_autotune", "mf_batch_autotune",
+    "mf_autotune", "mf_batch_autotune", "mf_graph_read_layout",
 ]
 
 
@@ -376,11 +377,7 @@ def gen_bipartite(L: int, R: int, mean_deg: float = 8.0, seed: int = 1) -> Graph
     return _gen(_lib.mf_gen_bipartite, L, R, mean_deg, seed)
 
 
-def build_layout(g: Graph, layout: int) -> dict:
-    """Host residual layout arrays (ResidualGraph::buildFromCSRGraph) for layout tests."""
-    c = g.csr()
-    arr = _LayoutArrays()
-    _check(_lib.mf_build_layout(C.byref(c), layout, C.byref(arr)), "build_layout")
+def _layout_dict(arr, layout: int, m: int) -> dict:
     n, S = arr.n, arr.slots
 
     def cp(p, k):
@@ -390,9 +387,17 @@ def build_layout(g: Graph, layout: int) -> dict:
         out = {"roffsets": cp(arr.a, n + 1), "rdst": cp(arr.b, S), "flow_index": cp(arr.c, S)}
     else:
         out = {"offsets": cp(arr.a, n + 1), "dst": cp(arr.b, S), "rev": cp(arr.c, S),
-               "residual": cp(arr.d, S), "fslot": cp(arr.e, g.m)}
+               "residual": cp(arr.d, S), "fslot": cp(arr.e, m)}
     _lib.mf_layout_free(C.byref(arr))
     return out
+
+
+def build_layout(g: Graph, layout: int) -> dict:
+    """Host residual layout arrays (ResidualGraph::buildFromCSRGraph) for layout tests."""
+    c = g.csr()
+    arr = _LayoutArrays()
+    _check(_lib.mf_build_layout(C.byref(c), layout, C.byref(arr)), "build_layout")
+    return _layout_dict(arr, layout, g.m)
 
 
 def return_excess(g: Graph, flow: np.ndarray) -> np.ndarray:
@@ -487,7 +492,7 @@ class Context:
         c = g.csr()
         p = _vp()
         _check(_lib.mf_graph_upload(self._p, C.byref(c), layout, C.byref(p)), "graph_upload")
-        return DeviceGraph(self, p, g)
+        return DeviceGraph(self, p, g, layout)
 
     def solve(self, g: Graph, algo: int = ALGO_VC, layout: int = LAYOUT_BCSR, flow: bool = False,
               **opts):
@@ -546,8 +551,8 @@ class Context:
 class DeviceGraph:
     """Residual graph resident in HBM (mf_graph)."""
 
-    def __init__(self, ctx: Context, p, g: Graph):
-        self.ctx, self._p, self.graph = ctx, p, g
+    def __init__(self, ctx: Context, p, g: Graph, layout: int = LAYOUT_BCSR):
+        self.ctx, self._p, self.graph, self.layout = ctx, p, g, layout
 
     def solve(self, algo: int = ALGO_VC, **opts) -> Result:
         o = options(**opts)
@@ -574,6 +579,12 @@ class DeviceGraph:
         _check(_lib.mf_graph_read_cut(self.ctx._p, self._p,
                                       side.ctypes.data_as(C.POINTER(C.c_uint8))), "read_cut")
         return side
+
+    def read_layout(self) -> dict:
+        """The resident residual layout (built on the device), in build_layout's format."""
+        arr = _LayoutArrays()
+        _check(_lib.mf_graph_read_layout(self.ctx._p, self._p, C.byref(arr)), "read_layout")
+        return _layout_dict(arr, self.layout, self.graph.m)
 
     def read_state(self):
         h = np.zeros(self.graph.n, dtype=np.int32)

@@ -30,6 +30,7 @@
 #include <cstring>
 #include <vector>
 
+#include "build.h"
 #include "kernels.h"
 #include "mf_internal.h"
 #include "solver.h"
@@ -1153,6 +1154,7 @@ struct DevGraph {
   int *roff = nullptr;  // RCSR reverse offsets
   int2 *rdi = nullptr;  // RCSR {tail, arc}
   int *cap0 = nullptr;  // initial residual per slot (per arc for RCSR)
+  int *fslot = nullptr; // BCSR, BCSR2: forward slot of every arc
   // terminals
   int *src = nullptr, *snk = nullptr;
   long long *values = nullptr;
@@ -1194,7 +1196,7 @@ constexpr int ctl_cnt(int set, int q, int c) { return set + (q * kNumClasses + c
 static void free_dev(DevGraph *g) {
   void *ptrs[] = {g->off, g->dc,  g->rev, g->dst,    g->cr,    g->fdc,   g->bcf,
                   g->roff, g->rdi, g->cap0, g->src,   g->snk,   g->values, g->h,
-                  g->e,   g->lists, g->ctl, g->stats, g->vcls, g->vorder, g->stamp};
+                  g->e,   g->lists, g->ctl, g->stats, g->vcls, g->vorder, g->stamp, g->fslot};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -1270,54 +1272,74 @@ static mf_status upload(mf_context *ctx, const mf_csr *c, const int32_t *src, co
     if (sc > INT32_MAX) return MF_ERR_OVERFLOW;  // every excess must fit int32
   }
   hipStream_t st = ctx->stream;
+  if (2 * c->m > INT32_MAX) return MF_ERR_OVERFLOW;
   std::vector<int64_t> degree((size_t)c->n, 0);
   mf_status rc;
-  if (layout == MF_LAYOUT_RCSR) {
-    RcsrLayout R;
-    if ((rc = build_rcsr_layout(c, R)) != MF_OK) return rc;
-    G.S = c->m;
-    std::vector<int2> fdc((size_t)c->m), rdi((size_t)c->m);
-    for (int64_t i = 0; i < c->m; ++i) fdc[(size_t)i] = make_int2(c->dst[i], G.h_cap[(size_t)i]);
-    for (int64_t q = 0; q < c->m; ++q)
-      rdi[(size_t)q] = make_int2(R.rdst[(size_t)q], R.fidx[(size_t)q]);
-    if ((rc = dmalloc(&G.off, (size_t)c->n + 1)) || (rc = dmalloc(&G.fdc, (size_t)c->m)) ||
-        (rc = dmalloc(&G.bcf, (size_t)c->m)) || (rc = dmalloc(&G.roff, (size_t)c->n + 1)) ||
-        (rc = dmalloc(&G.rdi, (size_t)c->m)) || (rc = dmalloc(&G.cap0, (size_t)c->m)))
-      return rc;
-    HIP_TRY(hipMemcpyAsync(G.off, c->offsets, sizeof(int) * (c->n + 1), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(G.fdc, fdc.data(), sizeof(int2) * c->m, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(G.roff, R.roff.data(), sizeof(int) * (c->n + 1), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(G.rdi, rdi.data(), sizeof(int2) * c->m, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(G.cap0, G.h_cap.data(), sizeof(int) * c->m, hipMemcpyHostToDevice, st));
-    for (int32_t v = 0; v < c->n; ++v)
-      degree[(size_t)v] = (int64_t)(c->offsets[v + 1] - c->offsets[v]) +
-                          (R.roff[(size_t)v + 1] - R.roff[(size_t)v]);
-    HIP_TRY(hipStreamSynchronize(st));
-  } else {
-    SlotLayout B;
-    if ((rc = build_slot_layout(c, B)) != MF_OK) return rc;
-    G.S = B.slots;
-    G.h_fslot = std::move(B.fslot);
-    if ((rc = dmalloc(&G.off, (size_t)c->n + 1)) || (rc = dmalloc(&G.cap0, (size_t)G.S))) return rc;
-    HIP_TRY(hipMemcpyAsync(G.off, B.off.data(), sizeof(int) * (c->n + 1), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(G.cap0, B.cap0.data(), sizeof(int) * G.S, hipMemcpyHostToDevice, st));
-    if (layout == MF_LAYOUT_BCSR) {
-      std::vector<int2> dc((size_t)G.S);
-      for (int64_t j = 0; j < G.S; ++j) dc[(size_t)j] = make_int2(B.dst[(size_t)j], B.cap0[(size_t)j]);
-      if ((rc = dmalloc(&G.dc, (size_t)G.S)) || (rc = dmalloc(&G.rev, (size_t)G.S))) return rc;
-      HIP_TRY(hipMemcpyAsync(G.dc, dc.data(), sizeof(int2) * G.S, hipMemcpyHostToDevice, st));
-      HIP_TRY(hipMemcpyAsync(G.rev, B.rev.data(), sizeof(int) * G.S, hipMemcpyHostToDevice, st));
-      HIP_TRY(hipStreamSynchronize(st));
-    } else {
-      std::vector<int2> cr((size_t)G.S);
-      for (int64_t j = 0; j < G.S; ++j) cr[(size_t)j] = make_int2(B.cap0[(size_t)j], B.rev[(size_t)j]);
-      if ((rc = dmalloc(&G.dst, (size_t)G.S)) || (rc = dmalloc(&G.cr, (size_t)G.S))) return rc;
-      HIP_TRY(hipMemcpyAsync(G.dst, B.dst.data(), sizeof(int) * G.S, hipMemcpyHostToDevice, st));
-      HIP_TRY(hipMemcpyAsync(G.cr, cr.data(), sizeof(int2) * G.S, hipMemcpyHostToDevice, st));
-      HIP_TRY(hipStreamSynchronize(st));
-    }
-    for (int32_t v = 0; v < c->n; ++v) degree[(size_t)v] = B.off[(size_t)v + 1] - B.off[(size_t)v];
+  // the CSR goes to the device as it is; the residual layout is built there (build.hip)
+  int *d_off = nullptr, *d_dst = nullptr, *d_cap = nullptr;
+  auto free_raw = [&]() {
+    if (d_off && d_off != G.off) (void)hipFree(d_off);
+    if (d_dst) (void)hipFree(d_dst);
+    if (d_cap) (void)hipFree(d_cap);
+  };
+  if ((rc = dmalloc(&d_off, (size_t)c->n + 1)) || (rc = dmalloc(&d_dst, (size_t)c->m)) ||
+      (rc = dmalloc(&d_cap, (size_t)c->m))) {
+    free_raw();
+    return rc;
   }
+  if (hipMemcpyAsync(d_off, c->offsets, sizeof(int) * (c->n + 1), hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(d_dst, c->dst, sizeof(int) * c->m, hipMemcpyHostToDevice, st) ||
+      hipMemcpyAsync(d_cap, G.h_cap.data(), sizeof(int) * c->m, hipMemcpyHostToDevice, st)) {
+    free_raw();
+    return MF_ERR_DEVICE;
+  }
+  DeviceLayout out;
+  if (layout == MF_LAYOUT_RCSR) {
+    G.S = c->m;
+    G.off = d_off;  // RCSR keeps the arc offsets
+    if ((rc = dmalloc(&G.fdc, (size_t)c->m)) || (rc = dmalloc(&G.bcf, (size_t)c->m)) ||
+        (rc = dmalloc(&G.roff, (size_t)c->n + 1)) || (rc = dmalloc(&G.rdi, (size_t)c->m)) ||
+        (rc = dmalloc(&G.cap0, (size_t)c->m))) {
+      free_raw();
+      return rc;
+    }
+    out.fdc = G.fdc;
+    out.roff = G.roff;
+    out.rdi = G.rdi;
+    out.cap0 = G.cap0;
+  } else {
+    G.S = 2 * c->m;
+    if ((rc = dmalloc(&G.off, (size_t)c->n + 1)) || (rc = dmalloc(&G.cap0, (size_t)G.S)) ||
+        (rc = dmalloc(&G.fslot, (size_t)c->m))) {
+      free_raw();
+      return rc;
+    }
+    out.soff = G.off;
+    out.cap0 = G.cap0;
+    out.fslot = G.fslot;
+    if (layout == MF_LAYOUT_BCSR) {
+      rc = dmalloc(&G.dc, (size_t)G.S);
+      if (rc == MF_OK) rc = dmalloc(&G.rev, (size_t)G.S);
+      out.dc = G.dc;
+      out.rev = G.rev;
+    } else {
+      rc = dmalloc(&G.dst, (size_t)G.S);
+      if (rc == MF_OK) rc = dmalloc(&G.cr, (size_t)G.S);
+      out.dst2 = G.dst;
+      out.cr = G.cr;
+    }
+    if (rc != MF_OK) {
+      free_raw();
+      return rc;
+    }
+  }
+  std::vector<int> in_off((size_t)c->n + 1);
+  rc = build_layout_device(st, c->n, c->m, d_off, d_dst, d_cap, layout, out, in_off.data());
+  free_raw();
+  if (rc != MF_OK) return rc;
+  for (int32_t v = 0; v < c->n; ++v)
+    degree[(size_t)v] = (int64_t)(c->offsets[v + 1] - c->offsets[v]) +
+                        (in_off[(size_t)v + 1] - in_off[(size_t)v]);
   if ((rc = dmalloc(&G.h, (size_t)c->n)) || (rc = dmalloc(&G.e, (size_t)c->n)) ||
       (rc = dmalloc(&G.stamp, (size_t)c->n)) ||
       (rc = dmalloc(&G.lists, (size_t)4 * c->n)) || (rc = dmalloc(&G.ctl, (size_t)CTL_SIZE)) ||
@@ -1897,6 +1919,10 @@ mf_status mf_graph_read_flow(mf_context *ctx, mf_graph *mg, int32_t *flow_out,
   DevGraph &G = mg->g;
   if (!G.solved) return MF_ERR_INVALID_ARGUMENT;
   HIP_TRY(hipSetDevice(ctx->device));
+  if (G.layout != MF_LAYOUT_RCSR && G.h_fslot.empty() && G.m > 0) {
+    G.h_fslot.resize((size_t)G.m);
+    HIP_TRY(hipMemcpy(G.h_fslot.data(), G.fslot, sizeof(int) * G.m, hipMemcpyDeviceToHost));
+  }
   if (G.layout == MF_LAYOUT_RCSR) {
     std::vector<int2> fdc((size_t)G.m);
     HIP_TRY(hipMemcpy(fdc.data(), G.fdc, sizeof(int2) * G.m, hipMemcpyDeviceToHost));
@@ -1923,6 +1949,58 @@ mf_status mf_graph_read_flow(mf_context *ctx, mf_graph *mg, int32_t *flow_out,
     c.t = G.h_snk[0];
     return_excess_to_source(&c, flow_out);
   }
+  return MF_OK;
+}
+
+mf_status mf_graph_read_layout(mf_context *ctx, mf_graph *mg, mf_layout_arrays *out) {
+  if (!ctx || !mg || !out) return MF_ERR_INVALID_ARGUMENT;
+  memset(out, 0, sizeof(*out));
+  DevGraph &G = mg->g;
+  HIP_TRY(hipSetDevice(ctx->device));
+  auto alloc = [](int64_t k) { return (int32_t *)malloc(sizeof(int32_t) * (size_t)(k > 0 ? k : 1)); };
+  auto pairs = [&](const int2 *d, int64_t k, int32_t *x, int32_t *y) -> hipError_t {
+    std::vector<int2> h((size_t)k);
+    hipError_t err = hipMemcpy(h.data(), d, sizeof(int2) * (size_t)k, hipMemcpyDeviceToHost);
+    for (int64_t j = 0; j < k; ++j) {
+      if (x) x[j] = h[(size_t)j].x;
+      if (y) y[j] = h[(size_t)j].y;
+    }
+    return err;
+  };
+  out->n = G.n;
+  if (G.layout == MF_LAYOUT_RCSR) {
+    out->slots = G.m;
+    out->a = alloc((int64_t)G.n + 1);
+    out->b = alloc(G.m);
+    out->c = alloc(G.m);
+    if (!out->a || !out->b || !out->c) {
+      mf_layout_free(out);
+      return MF_ERR_OUT_OF_MEMORY;
+    }
+    HIP_TRY(hipMemcpy(out->a, G.roff, sizeof(int) * (G.n + 1), hipMemcpyDeviceToHost));
+    HIP_TRY(pairs(G.rdi, G.m, out->b, out->c));
+    return MF_OK;
+  }
+  out->slots = G.S;
+  out->a = alloc((int64_t)G.n + 1);
+  out->b = alloc(G.S);
+  out->c = alloc(G.S);
+  out->d = alloc(G.S);
+  out->e = alloc(G.m);
+  if (!out->a || !out->b || !out->c || !out->d || !out->e) {
+    mf_layout_free(out);
+    return MF_ERR_OUT_OF_MEMORY;
+  }
+  HIP_TRY(hipMemcpy(out->a, G.off, sizeof(int) * (G.n + 1), hipMemcpyDeviceToHost));
+  if (G.layout == MF_LAYOUT_BCSR) {
+    HIP_TRY(pairs(G.dc, G.S, out->b, nullptr));
+    HIP_TRY(hipMemcpy(out->c, G.rev, sizeof(int) * G.S, hipMemcpyDeviceToHost));
+  } else {
+    HIP_TRY(hipMemcpy(out->b, G.dst, sizeof(int) * G.S, hipMemcpyDeviceToHost));
+    HIP_TRY(pairs(G.cr, G.S, nullptr, out->c));
+  }
+  HIP_TRY(hipMemcpy(out->d, G.cap0, sizeof(int) * G.S, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out->e, G.fslot, sizeof(int) * G.m, hipMemcpyDeviceToHost));
   return MF_OK;
 }
 
