@@ -107,7 +107,10 @@ typedef struct mf_options {
                                  it in the same cycle, up to this many vertices per cycle;
                                  0 = default (2), -1 = off (every activated vertex waits for the
                                  next cycle) */
-  int32_t reserved[4];
+  int32_t record_workload;    /* 1: sum the busy clock cycles of every wave of the push-relabel
+                                 launches (WORKLOAD of maxflow-cuda/src/push_relabel_kernel.cu:25-40,
+                                 146-152); read with mf_graph_read_workload */
+  int32_t reserved[3];
 } mf_options;
 
 typedef struct mf_result {
@@ -212,6 +215,13 @@ mf_status mf_graph_read_cut(mf_context *ctx, mf_graph *g, uint8_t *sink_side);
  * mf_layout_free).  mf_graph_upload builds it on the device; the two builders give identical
  * arrays. */
 mf_status mf_graph_read_layout(mf_context *ctx, mf_graph *g, mf_layout_arrays *out);
+/* Per-wave busy clock cycles of the last solve run with record_workload (the per-warp execution
+ * times maxflow-cuda prints after "Warp execution time:", maxflow-cuda/src/push_relabel.cu:202-218;
+ * a wave here is a 64-lane wavefront) and the degree class of each wave's block.  *num_waves is
+ * set (0 if nothing was recorded); cycles / wave_class (either may be NULL) need capacity >=
+ * *num_waves entries. */
+mf_status mf_graph_read_workload(mf_context *ctx, mf_graph *g, uint64_t *cycles,
+                                 uint8_t *wave_class, int64_t capacity, int64_t *num_waves);
 /* Heights and excesses after a solve (n entries each; either may be NULL). */
 mf_status mf_graph_read_state(mf_context *ctx, mf_graph *g, int32_t *height, int32_t *excess);
 mf_status mf_graph_destroy(mf_context *ctx, mf_graph *g);

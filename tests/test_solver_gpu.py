@@ -310,3 +310,32 @@ def test_device_layout_matches_host_builder(mf, ctx, golden, name, layout):
     assert host.keys() == dev.keys()
     for k in host:
         assert np.array_equal(host[k], dev[k]), k
+
+
+def test_cli_workload_report(golden):
+    # the WORKLOAD report that bash_run.py:21-22 parses: "#warps: N", then N integers
+    path = os.path.join(GOLDEN, "m50.max")
+    r = subprocess.run([os.path.join(ROOT, "wbpr_amd", "bin", "maxflow"), "-v", "2", "-f", path,
+                        "-a", "1", "-w"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    i = next(k for k, ln in enumerate(lines) if ln.startswith("#warps:"))
+    nw = int(lines[i].split()[1])
+    assert lines[i + 1].startswith("Warp execution time:")
+    vals = [int(x) for x in lines[i + 2].split()]
+    assert nw > 0 and len(vals) == nw and sum(vals) > 0
+
+
+def test_workload_record(mf, ctx, golden):
+    g, value = generated_graph(mf, golden, "rmat12_s3")
+    dg = ctx.upload(g, mf.LAYOUT_BCSR)
+    try:
+        r = dg.solve(mf.ALGO_VC, record_workload=1)
+        assert r.value == value
+        cyc, cls = dg.read_workload()
+        assert len(cyc) > 0 and len(cyc) % 4 == 0 and cyc.sum() > 0
+        assert cls.max() < 5
+        r2 = dg.solve(mf.ALGO_VC)  # without recording the last record stays readable
+        assert r2.value == value and len(dg.read_workload()[0]) == len(cyc)
+    finally:
+        dg.close()

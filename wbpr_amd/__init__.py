@@ -101,7 +101,8 @@ class _Options(C.Structure):
         ("class_lanes", C.c_uint16 * 5),
         ("pad0", C.c_uint16),
         ("forward_hops", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("record_workload", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
     def to_dict(self) -> dict:
@@ -202,6 +203,8 @@ _sig("mf_graph_read_flow", C.c_int, _vp, _vp, _i32p, C.c_int32)
 _sig("mf_graph_read_cut", C.c_int, _vp, _vp, C.POINTER(C.c_uint8))
 _sig("mf_graph_read_state", C.c_int, _vp, _vp, _i32p, _i32p)
 _sig("mf_graph_read_layout", C.c_int, _vp, _vp, C.POINTER(_LayoutArrays))
+_sig("mf_graph_read_workload", C.c_int, _vp, _vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint8),
+     C.c_int64, C.POINTER(C.c_int64))
 _sig("mf_graph_destroy", C.c_int, _vp, _vp)
 _sig("mf_solve", C.c_int, _vp, C.POINTER(_Csr), C.c_int, C.c_int, C.POINTER(_Options),
      C.POINTER(_Result), _i32p)
@@ -234,7 +237,7 @@ EXPORTED = [
     "mf_graph_destroy", "mf_solve", "mf_rcsr_push_relabel", "mf_batch_solve",
     "mf_server_create", "mf_server_submit", "mf_server_progress", "mf_server_result",
     "mf_server_stats_get", "mf_server_destroy", "mf_footprint_bytes", "mf_flow_return_excess",
-    "mf_autotune", "mf_batch_autotune", "mf_graph_read_layout",
+    "mf_autotune", "mf_batch_autotune", "mf_graph_read_layout", "mf_graph_read_workload",
 ]
 
 
@@ -585,6 +588,20 @@ class DeviceGraph:
         arr = _LayoutArrays()
         _check(_lib.mf_graph_read_layout(self.ctx._p, self._p, C.byref(arr)), "read_layout")
         return _layout_dict(arr, self.layout, self.graph.m)
+
+    def read_workload(self):
+        """Per-wave busy cycles of the last solve run with record_workload=1, and the degree
+        class of each wave's block (empty arrays when nothing was recorded)."""
+        n = C.c_int64()
+        _check(_lib.mf_graph_read_workload(self.ctx._p, self._p, None, None, 0, C.byref(n)),
+               "read_workload")
+        cyc = np.zeros(max(n.value, 1), dtype=np.uint64)
+        cls = np.zeros(max(n.value, 1), dtype=np.uint8)
+        if n.value:
+            _check(_lib.mf_graph_read_workload(
+                self.ctx._p, self._p, cyc.ctypes.data_as(C.POINTER(C.c_uint64)),
+                cls.ctypes.data_as(C.POINTER(C.c_uint8)), n.value, C.byref(n)), "read_workload")
+        return cyc[: n.value], cls[: n.value]
 
     def read_state(self):
         h = np.zeros(self.graph.n, dtype=np.int32)

@@ -38,6 +38,7 @@ static void usage(const char *prog) {
   printf("\t-k cycles\tPush-relabel cycles between global relabels (default automatic)\n");
   printf("\t-c\t\tCheck the result: capacity, conservation and min-cut certificate\n");
   printf("\t-o file\t\tWrite the flow in DIMACS solution format (s/f lines)\n");
+  printf("\t-w\t\tPrint the busy cycles of every wave of the push-relabel launches\n");
 }
 
 // Graph statistics in the format of CSRGraph::printGraphStatus (maxflow-cuda/src/graph.cpp:668-683).
@@ -107,11 +108,11 @@ static int check_certificate(const mf_csr *c, const int32_t *flow, long long val
 }
 
 int main(int argc, char **argv) {
-  int ch, file_type = 0, algo_type = 0, layout = 1, cycles = 0, check = 0;
+  int ch, file_type = 0, algo_type = 0, layout = 1, cycles = 0, check = 0, workload = 0;
   int source = -1, sink = -1;
   const char *filename = nullptr, *outfile = nullptr;
   mf_host_graph *g = nullptr;
-  while ((ch = getopt(argc, argv, "hv:f:s:t:a:l:k:co:")) != -1) {
+  while ((ch = getopt(argc, argv, "hv:f:s:t:a:l:k:co:w")) != -1) {
     switch (ch) {
       case 'h':
         usage(argv[0]);
@@ -168,6 +169,9 @@ int main(int argc, char **argv) {
       case 'o':
         outfile = optarg;
         break;
+      case 'w':
+        workload = 1;
+        break;
       default:
         printf("Invalid option\n");
         return 1;
@@ -212,6 +216,7 @@ int main(int argc, char **argv) {
   mf_options o;
   mf_options_default(&o);
   o.cycles_per_relabel = cycles;
+  o.record_workload = workload;
   mf_graph *dg = nullptr;
   st = mf_graph_upload(ctx, &c, (mf_layout)layout, &dg);
   mf_result r;
@@ -232,6 +237,19 @@ int main(int argc, char **argv) {
          "implementation is %lld, %lld\n",
          (long long)r.value, (long long)r.value);
   int rc = 0;
+  if (workload) {  // the WORKLOAD report of maxflow-cuda/src/push_relabel.cu:202-210
+    int64_t nw = 0;
+    std::vector<uint64_t> cyc;
+    if (mf_graph_read_workload(ctx, dg, nullptr, nullptr, 0, &nw) == MF_OK && nw > 0) {
+      cyc.resize((size_t)nw);
+      if (mf_graph_read_workload(ctx, dg, cyc.data(), nullptr, nw, &nw) != MF_OK) nw = 0;
+    }
+    printf("------------<< Workload Information >>------------\n");
+    printf("#warps: %lld\n", (long long)nw);
+    printf("Warp execution time:\n");
+    for (int64_t i = 0; i < nw; ++i) printf("%llu ", (unsigned long long)cyc[(size_t)i]);
+    printf("\n");
+  }
   if (check || outfile) {
     std::vector<int32_t> flow((size_t)c.m);
     std::vector<uint8_t> side((size_t)c.n);

@@ -55,6 +55,8 @@ struct CycleArgs {
   int fwd_hops;  // push forwarding: further vertices a tile may discharge in the same cycle
   int *stamp;    // per vertex: the last cycle that discharged it
   int cycle;     // this cycle's number (increasing within a solve)
+  unsigned long long *work;  // optional: busy clock cycles of every wave, summed over launches
+  unsigned char *work_cls;   // optional: degree class of every wave's block
 };
 
 // Per-lane scan state: the lane's lowest (height, key) over its residual slots and, for that
@@ -326,6 +328,7 @@ template <class L>
 __global__ __launch_bounds__(kBlock) void k_cycle(L g, CycleArgs a, Dispatch d) {
   // counters of the list two cycles ahead: last read by the previous launch, next written by
   // the following one
+  const long long t_start = a.work ? clock64() : 0;
   if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses)
     a.clear_cnt[threadIdx.x * kCtlStride] = 0;
   __shared__ Stage S;
@@ -345,6 +348,11 @@ __global__ __launch_bounds__(kBlock) void k_cycle(L g, CycleArgs a, Dispatch d) 
     case 32: cycle_class<32>(g, a, &S, c, cnt, b, nblk, st); break;
     case 64: cycle_class<64>(g, a, &S, c, cnt, b, nblk, st); break;
     default: cycle_class<kBlock>(g, a, &S, c, cnt, b, nblk, st); break;
+  }
+  if (a.work && __lane_id() == 0) {  // the wave's own work, before the block's flush
+    const int wv = (int)blockIdx.x * (kBlock / 64) + (int)threadIdx.x / 64;
+    a.work[wv] += (unsigned long long)(clock64() - t_start);  // one writer per wave slot
+    a.work_cls[wv] = (unsigned char)c;
   }
   stage_flush(S, a.out, a.nclass);
   block_flush(st.active, st.slots, st.pushes, st.relabels, a.stats, ST_ACTIVE);
@@ -873,6 +881,8 @@ __device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, i
   ca.stats = nullptr;
   ca.fwd_hops = b.fwd_hops;
   ca.stamp = b.stamp;
+  ca.work = nullptr;
+  ca.work_cls = nullptr;
   ca.cycle = 0;
 
   // preflow: saturate the source's residual arcs
@@ -1155,6 +1165,9 @@ struct DevGraph {
   int2 *rdi = nullptr;  // RCSR {tail, arc}
   int *cap0 = nullptr;  // initial residual per slot (per arc for RCSR)
   int *fslot = nullptr; // BCSR, BCSR2: forward slot of every arc
+  unsigned long long *work = nullptr;  // record_workload: busy cycles per wave of k_cycle
+  unsigned char *work_cls = nullptr;   //   and the degree class of each wave's block
+  int64_t work_waves = 0;
   // terminals
   int *src = nullptr, *snk = nullptr;
   long long *values = nullptr;
@@ -1196,7 +1209,8 @@ constexpr int ctl_cnt(int set, int q, int c) { return set + (q * kNumClasses + c
 static void free_dev(DevGraph *g) {
   void *ptrs[] = {g->off, g->dc,  g->rev, g->dst,    g->cr,    g->fdc,   g->bcf,
                   g->roff, g->rdi, g->cap0, g->src,   g->snk,   g->values, g->h,
-                  g->e,   g->lists, g->ctl, g->stats, g->vcls, g->vorder, g->stamp, g->fslot};
+                  g->e,   g->lists, g->ctl, g->stats, g->vcls, g->vorder, g->stamp, g->fslot,
+                  g->work, g->work_cls};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -1430,6 +1444,26 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   ca.stats = G.stats;
   ca.fwd_hops = o.forward_hops < 0 ? 0 : (o.forward_hops > 0 ? o.forward_hops : kDefaultForwardHops);
   ca.stamp = G.stamp;
+  ca.work = nullptr;
+  ca.work_cls = nullptr;
+  if (o.record_workload) {  // per-wave busy cycles, summed over this solve's cycle launches
+    const int64_t waves = (int64_t)grid * (kBlock / 64);
+    if (G.work_waves != waves) {
+      if (G.work) (void)hipFree(G.work);
+      if (G.work_cls) (void)hipFree(G.work_cls);
+      G.work = nullptr;
+      G.work_cls = nullptr;
+      G.work_waves = 0;
+      mf_status rc;
+      if ((rc = dmalloc(&G.work, (size_t)waves)) || (rc = dmalloc(&G.work_cls, (size_t)waves)))
+        return rc;
+      G.work_waves = waves;
+    }
+    HIP_TRY(hipMemsetAsync(G.work, 0, sizeof(unsigned long long) * waves, st));
+    HIP_TRY(hipMemsetAsync(G.work_cls, 0, (size_t)waves, st));
+    ca.work = G.work;
+    ca.work_cls = G.work_cls;
+  }
 
   const int fmax = o.bfs_block_frontier > 0 ? o.bfs_block_frontier : kDefaultBfsFrontier;
   const int fmax_low = std::max(1, fmax / 2);
@@ -1949,6 +1983,22 @@ mf_status mf_graph_read_flow(mf_context *ctx, mf_graph *mg, int32_t *flow_out,
     c.t = G.h_snk[0];
     return_excess_to_source(&c, flow_out);
   }
+  return MF_OK;
+}
+
+mf_status mf_graph_read_workload(mf_context *ctx, mf_graph *mg, uint64_t *cycles,
+                                 uint8_t *wave_class, int64_t capacity, int64_t *num_waves) {
+  if (!ctx || !mg || !num_waves) return MF_ERR_INVALID_ARGUMENT;
+  DevGraph &G = mg->g;
+  *num_waves = G.work_waves;
+  if (!G.work || (!cycles && !wave_class)) return MF_OK;
+  if (capacity < G.work_waves) return MF_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (cycles)
+    HIP_TRY(hipMemcpy(cycles, G.work, sizeof(uint64_t) * (size_t)G.work_waves,
+                      hipMemcpyDeviceToHost));
+  if (wave_class)
+    HIP_TRY(hipMemcpy(wave_class, G.work_cls, (size_t)G.work_waves, hipMemcpyDeviceToHost));
   return MF_OK;
 }
 
