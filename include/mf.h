@@ -166,6 +166,23 @@ mf_status mf_save_snap_bin(const char *path, const mf_csr *g);
 mf_status mf_load_dimacs(const char *path, mf_host_graph **out);
 /* DIMACS writer ('p max', 'n', 'a' lines), 1-based. */
 mf_status mf_save_dimacs(const char *path, const mf_csr *g);
+/* Front-ends of the paper's evaluation (WBPR_HPEC2025_CameraReady.pdf p.4, "Applications and
+ * datasets"), which the reference snapshot does not ship:
+ *   mf_load_konect_bipartite  KONECT bipartite edge list ('%' comments, "u v ..." with 1-based
+ *                             left and right ids) -> matching network: source 0 -> left 1..L,
+ *                             left -> right L+1..L+R, right -> sink L+R+1, capacities 1;
+ *   mf_super_terminals        super-source n -> the given sources, the given sinks -> super-sink
+ *                             n+1, with arc capacities that never bind (total out-capacity of a
+ *                             source, total in-capacity of a sink);
+ *   mf_bfs_terminal_pairs     `pairs` disjoint source/sink pairs, each sink drawn (seeded) from the
+ *                             farthest quarter of a BFS from its source, then mf_super_terminals.
+ *                             sources_out / sinks_out (pairs entries each, optional) receive them;
+ *                             MF_ERR_NO_PATH if not enough pairs are found. */
+mf_status mf_load_konect_bipartite(const char *path, mf_host_graph **out);
+mf_status mf_super_terminals(const mf_csr *g, const int32_t *sources, int32_t ns,
+                             const int32_t *sinks, int32_t nt, mf_host_graph **out);
+mf_status mf_bfs_terminal_pairs(const mf_csr *g, int32_t pairs, uint64_t seed,
+                                int32_t *sources_out, int32_t *sinks_out, mf_host_graph **out);
 void mf_host_graph_free(mf_host_graph *g);
 mf_csr mf_host_graph_view(const mf_host_graph *g);
 

@@ -262,3 +262,91 @@ def test_cli_errors(tmp_path):
     assert r.returncode == 1 and "Invalid algorithm type" in r.stdout
     r = run_cli("-v", "0", "-f", os.path.join(GOLDEN, "snap_4.txt"), "-s", "2", "-t", "2")
     assert r.returncode == 1
+
+
+KONECT_SMALL = """% bip unweighted
+% 9 4 3
+1 1
+1 2
+2 1
+3 2
+3 3
+4 3
+4 3
+2 2
+4 1
+"""
+
+
+def test_konect_bipartite_loader(mf, orc, tmp_path):
+    # the paper's bipartite-matching inputs: super-source -> L, L -> R, R -> super-sink
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import maximum_bipartite_matching
+    p = tmp_path / "out.small"
+    p.write_text(KONECT_SMALL)
+    g = mf.load_konect_bipartite(str(p))
+    L, R = 4, 3
+    assert g.n == L + R + 2 and g.s == 0 and g.t == L + R + 1 and g.m_header == 9
+    assert g.m == L + 9 + R and set(g.cap.tolist()) == {1}
+    tails = g.tails()
+    assert sorted(g.dst[tails == 0].tolist()) == [1, 2, 3, 4]
+    assert np.all(g.dst[(tails >= L + 1) & (tails <= L + R)] == g.t)
+    rows = [int(x.split()[0]) - 1 for x in KONECT_SMALL.splitlines() if x and x[0] != "%"]
+    cols = [int(x.split()[1]) - 1 for x in KONECT_SMALL.splitlines() if x and x[0] != "%"]
+    bi = sp.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(L, R))
+    matching = int((maximum_bipartite_matching(bi, perm_type="column") >= 0).sum())
+    assert orc.dinic_maxflow(g) == matching == 3
+    assert mf.load(str(p), 3).m == g.m
+
+
+def test_konect_errors(mf, tmp_path):
+    p = tmp_path / "bad"
+    p.write_text("% bip\n1 x\n")
+    with pytest.raises(mf.MFError) as ei:
+        mf.load_konect_bipartite(str(p))
+    assert ei.value.name == "MF_ERR_PARSE"
+    with pytest.raises(mf.MFError):
+        mf.load_konect_bipartite(str(tmp_path / "missing"))
+
+
+def test_super_terminals(mf, orc):
+    g = mf.gen_rmat(10, 8, seed=3)
+    h = mf.super_terminals(g, [1, 2, 3], [7, 8])
+    assert h.n == g.n + 2 and h.s == g.n and h.t == g.n + 1 and h.m == g.m + 5
+    tails = h.tails()
+    out_cap = np.bincount(g.tails(), weights=g.cap, minlength=g.n)
+    in_cap = np.bincount(g.dst, weights=g.cap, minlength=g.n)
+    sup = tails == h.s
+    assert sorted(h.dst[sup].tolist()) == [1, 2, 3]
+    for v, c in zip(h.dst[sup], h.cap[sup]):
+        assert c == out_cap[v]
+    for u in (7, 8):
+        sel = (tails == u) & (h.dst == h.t)
+        assert sel.sum() == 1 and h.cap[sel][0] == in_cap[u]
+    # the super arcs never bind: the value is the max flow between the terminal sets
+    assert orc.dinic_maxflow(h) <= min(out_cap[[1, 2, 3]].sum(), in_cap[[7, 8]].sum())
+    with pytest.raises(mf.MFError):
+        mf.super_terminals(g, [1, 1], [7])  # a vertex used twice
+    with pytest.raises(mf.MFError):
+        mf.super_terminals(g, [1], [g.n])  # out of range
+
+
+def test_bfs_terminal_pairs(mf):
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import shortest_path
+    g = mf.gen_rmat(11, 8, seed=4)
+    h, src, snk = mf.bfs_terminal_pairs(g, 20, seed=9)
+    h2, src2, snk2 = mf.bfs_terminal_pairs(g, 20, seed=9)
+    assert np.array_equal(src, src2) and np.array_equal(snk, snk2) and h.m == h2.m
+    assert len(set(src.tolist()) | set(snk.tolist())) == 40  # disjoint pairs
+    assert h.n == g.n + 2 and h.s == g.n and h.t == g.n + 1
+    A = csr_matrix((np.ones(g.m), (g.tails(), g.dst)), shape=(g.n, g.n))
+    d = shortest_path(A, unweighted=True, indices=src)
+    for i, (s, t) in enumerate(zip(src, snk)):
+        di = d[i][np.isfinite(d[i])]
+        dmax = int(di.max())
+        assert np.isfinite(d[i][t]) and d[i][t] >= max(dmax - dmax // 4, 1)
+    empty = mf.Graph.from_arcs(10, np.array([0]), np.array([1]), np.array([1]), 0, 1)
+    with pytest.raises(mf.MFError) as ei:
+        mf.bfs_terminal_pairs(empty, 3, seed=1)
+    assert ei.value.name == "MF_ERR_NO_PATH"

@@ -339,3 +339,36 @@ def test_workload_record(mf, ctx, golden):
         assert r2.value == value and len(dg.read_workload()[0]) == len(cyc)
     finally:
         dg.close()
+
+
+def test_front_ends_solve(mf, orc, ctx, tmp_path):
+    # the paper's two front-ends end to end: a SNAP-style graph with BFS-chosen terminal pairs
+    # joined to a super-source/sink, and a KONECT-style bipartite matching network
+    g = mf.gen_rmat(12, 8, seed=6)
+    h, _, _ = mf.bfs_terminal_pairs(g, 20, seed=3)
+    value = orc.dinic_maxflow(h)
+    for algo in ALGOS:
+        solve_and_certify(mf, orc, ctx, h, algo, "bcsr", value)
+    rng = np.random.default_rng(8)
+    lines = ["% bip unweighted", "% 6000 1500 1200"]
+    lines += [f"{rng.integers(1, 1501)} {rng.integers(1, 1201)}" for _ in range(6000)]
+    p = tmp_path / "out.konect"
+    p.write_text("\n".join(lines) + "\n")
+    b = mf.load_konect_bipartite(str(p))
+    value = orc.dinic_maxflow(b)
+    for algo in ALGOS:
+        solve_and_certify(mf, orc, ctx, b, algo, "rcsr", value)
+
+
+def test_cli_konect_and_pairs(tmp_path):
+    exe = os.path.join(ROOT, "wbpr_amd", "bin", "maxflow")
+    p = tmp_path / "out.k"
+    p.write_text("% bip unweighted\n% 5 3 3\n1 1\n2 1\n2 2\n3 3\n3 2\n")
+    r = subprocess.run([exe, "-v", "3", "-f", str(p), "-a", "1", "-c"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert re.search(r"implementation is 3, 3", r.stdout) and "Certificate: ok" in r.stdout
+    r = subprocess.run([exe, "-v", "0", "-f", os.path.join(GOLDEN, "snap_8.txt"), "-m", "2",
+                        "-c"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Super-source 8 and super-sink 9" in r.stdout and "Certificate: ok" in r.stdout

@@ -187,6 +187,10 @@ _sig("mf_device_info", C.c_int, _vp, _i32p, C.c_char_p, C.c_size_t)
 _sig("mf_load_snap_txt", C.c_int, C.c_char_p, _hgpp)
 _sig("mf_load_snap_bin", C.c_int, C.c_char_p, _hgpp)
 _sig("mf_load_dimacs", C.c_int, C.c_char_p, _hgpp)
+_sig("mf_load_konect_bipartite", C.c_int, C.c_char_p, _hgpp)
+_sig("mf_super_terminals", C.c_int, C.POINTER(_Csr), _i32p, C.c_int32, _i32p, C.c_int32, _hgpp)
+_sig("mf_bfs_terminal_pairs", C.c_int, C.POINTER(_Csr), C.c_int32, C.c_uint64, _i32p, _i32p,
+     _hgpp)
 _sig("mf_save_snap_bin", C.c_int, C.c_char_p, C.POINTER(_Csr))
 _sig("mf_save_dimacs", C.c_int, C.c_char_p, C.POINTER(_Csr))
 _sig("mf_host_graph_free", None, C.POINTER(_HostGraph))
@@ -238,6 +242,7 @@ EXPORTED = [
     "mf_server_create", "mf_server_submit", "mf_server_progress", "mf_server_result",
     "mf_server_stats_get", "mf_server_destroy", "mf_footprint_bytes", "mf_flow_return_excess",
     "mf_autotune", "mf_batch_autotune", "mf_graph_read_layout", "mf_graph_read_workload",
+    "mf_load_konect_bipartite", "mf_super_terminals", "mf_bfs_terminal_pairs",
 ]
 
 
@@ -352,9 +357,39 @@ def load_dimacs(path: str) -> Graph:
     return _load(_lib.mf_load_dimacs, path, "load_dimacs")
 
 
+def load_konect_bipartite(path: str) -> Graph:
+    """KONECT bipartite edge list -> unit-capacity matching network (the paper's p.4 setup)."""
+    return _load(_lib.mf_load_konect_bipartite, path, "load_konect_bipartite")
+
+
 def load(path: str, file_type: int) -> Graph:
-    """-v flag of the reference CLI: 0 SNAP txt, 1 SNAP binary, 2 DIMACS."""
-    return {0: load_snap_txt, 1: load_snap_bin, 2: load_dimacs}[file_type](path)
+    """-v flag of the CLI: 0 SNAP txt, 1 SNAP binary, 2 DIMACS (the reference's), 3 KONECT
+    bipartite."""
+    return {0: load_snap_txt, 1: load_snap_bin, 2: load_dimacs,
+            3: load_konect_bipartite}[file_type](path)
+
+
+def super_terminals(g: Graph, sources, sinks) -> Graph:
+    """Super-source n -> sources, sinks -> super-sink n+1 (capacities that never bind)."""
+    c = g.csr()
+    src = np.ascontiguousarray(sources, dtype=np.int32)
+    snk = np.ascontiguousarray(sinks, dtype=np.int32)
+    out = C.POINTER(_HostGraph)()
+    _check(_lib.mf_super_terminals(C.byref(c), _ptr(src), len(src), _ptr(snk), len(snk),
+                                   C.byref(out)), "super_terminals")
+    return _take(out)
+
+
+def bfs_terminal_pairs(g: Graph, pairs: int = 20, seed: int = 1):
+    """The paper's SNAP transform: `pairs` BFS-chosen source/sink pairs joined to a super-source
+    and super-sink.  Returns (graph, sources, sinks)."""
+    c = g.csr()
+    src = np.zeros(pairs, dtype=np.int32)
+    snk = np.zeros(pairs, dtype=np.int32)
+    out = C.POINTER(_HostGraph)()
+    _check(_lib.mf_bfs_terminal_pairs(C.byref(c), pairs, seed, _ptr(src), _ptr(snk),
+                                      C.byref(out)), "bfs_terminal_pairs")
+    return _take(out), src, snk
 
 
 def _gen(fn, *args) -> Graph:

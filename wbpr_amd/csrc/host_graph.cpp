@@ -500,6 +500,151 @@ mf_status mf_gen_bipartite(int32_t L, int32_t R, double mean_deg, uint64_t seed,
   return host_graph_from_arcs(n, arcs, source, sink, out);
 }
 
+// KONECT bipartite edge list (the paper's bipartite-matching inputs, p.4): '%' comment lines,
+// then "u v [weight [time]]" with 1-based ids in two separate id spaces (left u, right v).
+// Matching network: super-source 0 -> every left vertex 1..L, left u -> right L+v for every
+// edge, every right vertex -> super-sink L+R+1; all capacities 1.  L and R are the largest ids
+// seen (or the counts of a "% m L R" size line, when larger).
+mf_status mf_load_konect_bipartite(const char *path, mf_host_graph **out) {
+  if (!path || !out) return MF_ERR_INVALID_ARGUMENT;
+  std::string buf;
+  if (!read_whole_file(path, buf)) return MF_ERR_IO;
+  const char *p = buf.data(), *end = p + buf.size();
+  std::vector<int32_t> eu, ev;
+  int64_t L = 0, R = 0;
+  int comment_lines = 0;
+  while (p < end) {
+    const char *eol = (const char *)memchr(p, '\n', (size_t)(end - p));
+    if (!eol) eol = end;
+    const char *q = p;
+    while (q < eol && (*q == ' ' || *q == '\t' || *q == '\r')) ++q;
+    if (q < eol && *q == '%') {
+      // the second comment line of a KONECT file holds "m L R"
+      if (++comment_lines == 2) {
+        ++q;
+        int64_t a, b, c;
+        if (parse_int(q, eol, a) && parse_int(q, eol, b) && parse_int(q, eol, c)) {
+          L = std::max(L, b);
+          R = std::max(R, c);
+        }
+      }
+    } else if (q < eol) {
+      int64_t u, v;
+      if (!parse_int(q, eol, u) || !parse_int(q, eol, v)) return MF_ERR_PARSE;
+      if (u < 1 || v < 1 || u > INT32_MAX || v > INT32_MAX) return MF_ERR_PARSE;
+      eu.push_back((int32_t)u);
+      ev.push_back((int32_t)v);
+      L = std::max(L, u);
+      R = std::max(R, v);
+    }
+    p = eol + 1;
+  }
+  if (L + R + 2 > INT32_MAX) return MF_ERR_OVERFLOW;
+  const int32_t n = (int32_t)(L + R + 2), source = 0, sink = n - 1;
+  ArcList arcs;
+  for (int32_t l = 1; l <= L; ++l) arcs.add(source, l, 1);
+  for (size_t i = 0; i < eu.size(); ++i) arcs.add(eu[i], (int32_t)L + ev[i], 1);
+  for (int32_t r = 1; r <= R; ++r) arcs.add((int32_t)L + r, sink, 1);
+  mf_status st = host_graph_from_arcs(n, arcs, source, sink, out);
+  if (st != MF_OK) return st;
+  (*out)->m_header = (int64_t)eu.size();
+  (*out)->m_processed = (int64_t)eu.size();
+  return MF_OK;
+}
+
+// Super-source n and super-sink n+1 joined to the given sources and sinks (the paper's SNAP
+// transform, p.4).  The arc super-source -> s_i carries the total capacity out of s_i and the
+// arc t_i -> super-sink the total capacity into t_i, so neither limits the flow.
+mf_status mf_super_terminals(const mf_csr *g, const int32_t *sources, int32_t ns,
+                             const int32_t *sinks, int32_t nt, mf_host_graph **out) {
+  if (!g || !out || !sources || !sinks || ns < 1 || nt < 1) return MF_ERR_INVALID_ARGUMENT;
+  if (!g->offsets || !g->dst || g->n < 1 || g->m < 0) return MF_ERR_INVALID_ARGUMENT;
+  if ((int64_t)g->n + 2 > INT32_MAX) return MF_ERR_OVERFLOW;
+  std::vector<char> role((size_t)g->n, 0);
+  for (int32_t i = 0; i < ns; ++i) {
+    if (sources[i] < 0 || sources[i] >= g->n || role[(size_t)sources[i]]) return MF_ERR_INVALID_ARGUMENT;
+    role[(size_t)sources[i]] = 1;
+  }
+  for (int32_t i = 0; i < nt; ++i) {
+    if (sinks[i] < 0 || sinks[i] >= g->n || role[(size_t)sinks[i]]) return MF_ERR_INVALID_ARGUMENT;
+    role[(size_t)sinks[i]] = 2;
+  }
+  std::vector<int64_t> cin((size_t)g->n, 0), cout((size_t)g->n, 0);
+  ArcList arcs;
+  arcs.u.reserve((size_t)g->m + ns + nt);
+  for (int32_t u = 0; u < g->n; ++u)
+    for (int32_t a = g->offsets[u]; a < g->offsets[u + 1]; ++a) {
+      const int32_t c = g->cap ? g->cap[a] : 1;
+      arcs.add(u, g->dst[a], c);
+      cout[(size_t)u] += c;
+      cin[(size_t)g->dst[a]] += c;
+    }
+  const int32_t S = g->n, T = g->n + 1;
+  int64_t total = 0;
+  for (int32_t i = 0; i < ns; ++i) {
+    const int64_t c = cout[(size_t)sources[i]];
+    if (c > INT32_MAX) return MF_ERR_OVERFLOW;
+    total += c;
+    arcs.add(S, sources[i], (int32_t)c);
+  }
+  if (total > INT32_MAX) return MF_ERR_OVERFLOW;  // the super-source's excess must fit int32
+  for (int32_t i = 0; i < nt; ++i) {
+    const int64_t c = cin[(size_t)sinks[i]];
+    if (c > INT32_MAX) return MF_ERR_OVERFLOW;
+    arcs.add(sinks[i], T, (int32_t)c);
+  }
+  return host_graph_from_arcs(g->n + 2, arcs, S, T, out);
+}
+
+// Picks `pairs` source/sink pairs the way the paper describes for SNAP graphs (p.4): a BFS from
+// a random vertex, the sink drawn from the vertices at the longest quarter of the BFS distances
+// it reached; every vertex is used at most once.  Then joins them with mf_super_terminals.
+mf_status mf_bfs_terminal_pairs(const mf_csr *g, int32_t pairs, uint64_t seed,
+                                int32_t *sources_out, int32_t *sinks_out, mf_host_graph **out) {
+  if (!g || !out || pairs < 1 || !g->offsets || !g->dst || g->n < 2) return MF_ERR_INVALID_ARGUMENT;
+  const int32_t n = g->n;
+  Rng rng(seed);
+  std::vector<int32_t> dist((size_t)n), queue, srcs, snks, far;
+  std::vector<char> used((size_t)n, 0);
+  const int64_t tries = (int64_t)pairs * 64 + 256;
+  for (int64_t attempt = 0; attempt < tries && (int32_t)srcs.size() < pairs; ++attempt) {
+    const int32_t r = (int32_t)rng.range(0, n - 1);
+    if (used[(size_t)r] || g->offsets[r + 1] == g->offsets[r]) continue;
+    std::fill(dist.begin(), dist.end(), -1);
+    queue.assign(1, r);
+    dist[(size_t)r] = 0;
+    for (size_t h = 0; h < queue.size(); ++h) {
+      const int32_t u = queue[h];
+      for (int32_t a = g->offsets[u]; a < g->offsets[u + 1]; ++a) {
+        const int32_t w = g->dst[a];
+        if (dist[(size_t)w] < 0 && (!g->cap || g->cap[a] > 0)) {
+          dist[(size_t)w] = dist[(size_t)u] + 1;
+          queue.push_back(w);
+        }
+      }
+    }
+    if (queue.size() < 2) continue;
+    // distances of the reached vertices are non-decreasing along the queue: the last quarter of
+    // the distance range holds the farthest vertices
+    const int32_t dmax = dist[(size_t)queue.back()];
+    const int32_t dcut = dmax - dmax / 4;
+    far.clear();
+    for (size_t h = queue.size(); h-- > 1;) {
+      if (dist[(size_t)queue[h]] < std::max(dcut, 1)) break;
+      if (!used[(size_t)queue[h]]) far.push_back(queue[h]);
+    }
+    if (far.empty()) continue;
+    const int32_t t = far[(size_t)rng.range(0, (int64_t)far.size() - 1)];
+    used[(size_t)r] = used[(size_t)t] = 1;
+    srcs.push_back(r);
+    snks.push_back(t);
+  }
+  if ((int32_t)srcs.size() < pairs) return MF_ERR_NO_PATH;
+  if (sources_out) std::copy(srcs.begin(), srcs.end(), sources_out);
+  if (sinks_out) std::copy(snks.begin(), snks.end(), sinks_out);
+  return mf_super_terminals(g, srcs.data(), pairs, snks.data(), pairs, out);
+}
+
 }  // extern "C"
 
 namespace mf {

@@ -28,6 +28,7 @@ static void usage(const char *prog) {
   printf("\t\t\t\t0: SNAP txt file (default)\n");
   printf("\t\t\t\t1: SNAP binary file\n");
   printf("\t\t\t\t2: DIMACS file\n");
+  printf("\t\t\t\t3: KONECT bipartite edge list (matching network)\n");
   printf("\t-f filename\tSpecify the file path (binary or txt)\n");
   printf("\t-s source\tSource node\n");
   printf("\t-t sink\t\tSink node\n");
@@ -39,6 +40,7 @@ static void usage(const char *prog) {
   printf("\t-c\t\tCheck the result: capacity, conservation and min-cut certificate\n");
   printf("\t-o file\t\tWrite the flow in DIMACS solution format (s/f lines)\n");
   printf("\t-w\t\tPrint the busy cycles of every wave of the push-relabel launches\n");
+  printf("\t-m pairs\tSNAP input without -s/-t: super-source and super-sink over BFS-chosen pairs\n");
 }
 
 // Graph statistics in the format of CSRGraph::printGraphStatus (maxflow-cuda/src/graph.cpp:668-683).
@@ -108,11 +110,11 @@ static int check_certificate(const mf_csr *c, const int32_t *flow, long long val
 }
 
 int main(int argc, char **argv) {
-  int ch, file_type = 0, algo_type = 0, layout = 1, cycles = 0, check = 0, workload = 0;
+  int ch, file_type = 0, algo_type = 0, layout = 1, cycles = 0, check = 0, workload = 0, pairs = 0;
   int source = -1, sink = -1;
   const char *filename = nullptr, *outfile = nullptr;
   mf_host_graph *g = nullptr;
-  while ((ch = getopt(argc, argv, "hv:f:s:t:a:l:k:co:w")) != -1) {
+  while ((ch = getopt(argc, argv, "hv:f:s:t:a:l:k:co:wm:")) != -1) {
     switch (ch) {
       case 'h':
         usage(argv[0]);
@@ -137,6 +139,10 @@ int main(int argc, char **argv) {
           case 2:
             printf("Loading from DIMACS file: %s\n", filename);
             st = mf_load_dimacs(filename, &g);
+            break;
+          case 3:
+            printf("Loading from KONECT bipartite file: %s\n", filename);
+            st = mf_load_konect_bipartite(filename, &g);
             break;
           default:
             printf("Invalid file type\n");
@@ -172,6 +178,9 @@ int main(int argc, char **argv) {
       case 'w':
         workload = 1;
         break;
+      case 'm':
+        pairs = atoi(optarg);
+        break;
       default:
         printf("Invalid option\n");
         return 1;
@@ -181,9 +190,21 @@ int main(int argc, char **argv) {
     printf("No input file (-f)\n");
     return 1;
   }
-  if (file_type != 2) {
+  if (file_type < 2) {
     g->s = source;
     g->t = sink;
+  }
+  if (pairs > 0 && file_type < 2) {
+    mf_csr v = mf_host_graph_view(g);
+    mf_host_graph *h = nullptr;
+    const mf_status st = mf_bfs_terminal_pairs(&v, pairs, 1, nullptr, nullptr, &h);
+    if (st != MF_OK) {
+      printf("Cannot choose %d source/sink pairs: %s\n", pairs, mf_status_string(st));
+      return 1;
+    }
+    mf_host_graph_free(g);
+    g = h;
+    printf("Super-source %d and super-sink %d over %d BFS-chosen pairs\n", g->s, g->t, pairs);
   }
   print_graph_status(g);
   if (algo_type == 0) {
