@@ -110,7 +110,10 @@ typedef struct mf_options {
   int32_t record_workload;    /* 1: sum the busy clock cycles of every wave of the push-relabel
                                  launches (WORKLOAD of maxflow-cuda/src/push_relabel_kernel.cu:25-40,
                                  146-152); read with mf_graph_read_workload */
-  int32_t reserved[3];
+  int32_t cycle_graphs;       /* 1: issue the push-relabel cycles in chunks captured as hipGraphs
+                                 (replayed, one per chunk length, captured once per solve);
+                                 0: one launch per cycle */
+  int32_t reserved[2];
 } mf_options;
 
 typedef struct mf_result {
@@ -136,6 +139,7 @@ typedef struct mf_result {
   int32_t status;
   int32_t bfs_bottom_up_levels; /* BFS levels run bottom-up */
   int32_t reserved[6];
+  int64_t graph_launches;     /* cycle launches replayed from captured graphs */
 } mf_result;
 
 typedef struct mf_context mf_context;

@@ -372,3 +372,17 @@ def test_cli_konect_and_pairs(tmp_path):
                         "-c"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Super-source 8 and super-sink 9" in r.stdout and "Certificate: ok" in r.stdout
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("name", ["genrmf16_s1", "rlg64_s1", "rmat12_s3", "bip_3000_2000_s9"])
+def test_cycle_graphs(mf, orc, ctx, golden, name, layout):
+    # push-relabel chunks replayed from captured hipGraphs give the same values and certificates
+    g, value = generated_graph(mf, golden, name)
+    for algo in ALGOS:
+        r = solve_and_certify(mf, orc, ctx, g, algo, layout, value, cycle_graphs=1)
+        assert r.graph_launches > 0 or r.cycles == 0
+    # fixed cadences that are and are not whole graph periods, with timing and forwarding
+    for k in (5, 12, 37):
+        r = solve_and_certify(mf, orc, ctx, g, "vc", layout, value, cycle_graphs=1,
+                              cycles_per_relabel=k, profile_kernels=1, forward_hops=4)

@@ -102,7 +102,8 @@ class _Options(C.Structure):
         ("pad0", C.c_uint16),
         ("forward_hops", C.c_int32),
         ("record_workload", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("cycle_graphs", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
     def to_dict(self) -> dict:
@@ -112,7 +113,7 @@ class _Options(C.Structure):
                 "relabel_time_ratio": round(float(self.relabel_time_ratio), 4),
                 "batch_algo": self.batch_algo, "batch_mode": self.batch_mode,
                 "bfs_direction": self.bfs_direction, "class_lanes": list(self.class_lanes),
-                "forward_hops": self.forward_hops}
+                "forward_hops": self.forward_hops, "cycle_graphs": self.cycle_graphs}
 
 
 class _Result(C.Structure):
@@ -139,6 +140,7 @@ class _Result(C.Structure):
         ("status", C.c_int32),
         ("bfs_bottom_up_levels", C.c_int32),
         ("reserved", C.c_int32 * 6),
+        ("graph_launches", C.c_int64),
     ]
 
 
@@ -473,6 +475,7 @@ class Result:
     bfs_levels: int
     bfs_grid_levels: int
     bfs_bottom_up_levels: int
+    graph_launches: int
 
     @staticmethod
     def _from(r: _Result) -> "Result":

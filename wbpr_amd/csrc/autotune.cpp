@@ -10,6 +10,8 @@
 //                     inside one workgroup.
 //   forwarding        mf_options.forward_hops: vertices a tile discharges per cycle by following
 //                     its own pushes (long augmenting paths want many, churning graphs few).
+//   graph replay      mf_options.cycle_graphs (single instance): cycle chunks replayed from
+//                     captured hipGraphs, cached on the resident graph, or launched one by one.
 //   batch algorithm   mf_options.batch_algo (batched solves only): TC or VC cycles.
 //
 // Lock-free schedules differ from run to run, so a candidate's time is the best of two solves.
@@ -90,6 +92,11 @@ mf_status descend(mf_options cur, bool tune_lanes, bool tune_batch_algo, Run run
       std::memcpy(t.class_lanes, p, sizeof(t.class_lanes));
       if ((st = consider(t)) != MF_OK) return st;
     }
+  }
+  if (!tune_batch_algo) {  // single instance: cycle chunks as captured graphs or not
+    mf_options t = cur;
+    t.cycle_graphs = cur.cycle_graphs ? 0 : 1;
+    if ((st = consider(t)) != MF_OK) return st;
   }
   const int32_t h0 = cur.forward_hops;
   for (int32_t h : kForwardHops) {

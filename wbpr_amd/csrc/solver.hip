@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -54,10 +55,17 @@ struct CycleArgs {
   unsigned long long *stats;
   int fwd_hops;  // push forwarding: further vertices a tile may discharge in the same cycle
   int *stamp;    // per vertex: the last cycle that discharged it
-  int cycle;     // this cycle's number (increasing within a solve)
+  int cycle;     // this cycle's number (increasing within a solve); with cycle_base, the offset
+  const int *cycle_base;  // graph replays: the chunk's first cycle number, set before each replay
   unsigned long long *work;  // optional: busy clock cycles of every wave, summed over launches
   unsigned char *work_cls;   // optional: degree class of every wave's block
 };
+
+// This cycle's number; a launch replayed from a captured graph adds the chunk's first cycle
+// number, set in device memory before the replay.
+__device__ __forceinline__ int cycle_now(const CycleArgs &a) {
+  return a.cycle_base ? a.cycle + *a.cycle_base : a.cycle;
+}
 
 // Per-lane scan state: the lane's lowest (height, key) over its residual slots and, for that
 // slot, the neighbour and the residual read during the scan.
@@ -179,7 +187,7 @@ __device__ __forceinline__ int finish_vertex(const L &g, const CycleArgs &a, Sta
   // (too high), where a first discharge can only see values that are too low
   int fwd = -1;
   if (want_w && fcls >= 0 && (!a.vcls || a.vcls[mine.w] <= fcls) &&
-      atomicExch(&a.stamp[mine.w], a.cycle) != a.cycle) {
+      atomicExch(&a.stamp[mine.w], cycle_now(a)) != cycle_now(a)) {
     fwd = mine.w;
     want_w = false;
   }
@@ -291,8 +299,9 @@ __device__ __forceinline__ void discharge_block(const L &g, const CycleArgs &a, 
 template <int T>
 __device__ __forceinline__ bool claim_queued(const CycleArgs &a, int u, int lane) {
   int old = 0;
-  if (lane == 0) old = atomicExch(&a.stamp[u], a.cycle);
-  return __shfl(old, 0, T) != a.cycle;
+  const int cyc = cycle_now(a);
+  if (lane == 0) old = atomicExch(&a.stamp[u], cyc);
+  return __shfl(old, 0, T) != cyc;
 }
 
 template <int T, class L>
@@ -757,6 +766,8 @@ __global__ __launch_bounds__(kBlock) void k_preflow(L g, int *e, const int *src,
   block_flush(sum, 0, 0, 0, stats, ST_SOURCE_OUT);
 }
 
+__global__ void k_set_int(int *p, int v) { *p = v; }
+
 __global__ void k_gather_values(const int *e, const int *snk, int k, long long *out) {
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride) out[i] = e[snk[i]];
@@ -881,6 +892,7 @@ __device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, i
   ca.stats = nullptr;
   ca.fwd_hops = b.fwd_hops;
   ca.stamp = b.stamp;
+  ca.cycle_base = nullptr;
   ca.work = nullptr;
   ca.work_cls = nullptr;
   ca.cycle = 0;
@@ -1045,6 +1057,8 @@ constexpr int kWgBfsSlots = 32768;           // ... and up to about this many fr
 constexpr int kDefaultForwardHops = 2;       // push forwarding: vertices a tile chains per cycle
 constexpr int kCycleChunk = 64;              // most cycles launched between two worklist checks
 constexpr int kFirstChunk = 8;               // first chunk, before a cycle has been timed
+constexpr int kGraphPeriod = 6;              // cycles after which lists and counters repeat
+constexpr int kGraphChunkMax = 60;           // longest captured chunk
 // Direction of the grid-wide BFS levels (Beamer's rule): bottom-up once the frontier's slots
 // exceed 1/kBottomUpAlpha of the slots not yet explored, top-down again once the frontier holds
 // fewer than 1/kBottomUpBeta of the nonzero-degree vertices; never below kBottomUpMinVertices.
@@ -1168,6 +1182,13 @@ struct DevGraph {
   unsigned long long *work = nullptr;  // record_workload: busy cycles per wave of k_cycle
   unsigned char *work_cls = nullptr;   //   and the degree class of each wave's block
   int64_t work_waves = 0;
+  int *cycle_base = nullptr;  // graph replays: first cycle number of the replayed chunk
+  // captured cycle chunks, kept across solves: one set per configuration of the launches
+  struct CycleGraphs {
+    std::array<int, 9> key;
+    std::vector<hipGraphExec_t> exec;  // index: chunk length / kGraphPeriod
+  };
+  std::vector<CycleGraphs> cycle_graphs;
   // terminals
   int *src = nullptr, *snk = nullptr;
   long long *values = nullptr;
@@ -1207,10 +1228,14 @@ enum {
 constexpr int ctl_cnt(int set, int q, int c) { return set + (q * kNumClasses + c) * kCtlStride; }
 
 static void free_dev(DevGraph *g) {
+  for (auto &cg : g->cycle_graphs)
+    for (hipGraphExec_t x : cg.exec)
+      if (x) (void)hipGraphExecDestroy(x);
+  g->cycle_graphs.clear();
   void *ptrs[] = {g->off, g->dc,  g->rev, g->dst,    g->cr,    g->fdc,   g->bcf,
                   g->roff, g->rdi, g->cap0, g->src,   g->snk,   g->values, g->h,
                   g->e,   g->lists, g->ctl, g->stats, g->vcls, g->vorder, g->stamp, g->fslot,
-                  g->work, g->work_cls};
+                  g->work, g->work_cls, g->cycle_base};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -1355,7 +1380,7 @@ static mf_status upload(mf_context *ctx, const mf_csr *c, const int32_t *src, co
     degree[(size_t)v] = (int64_t)(c->offsets[v + 1] - c->offsets[v]) +
                         (in_off[(size_t)v + 1] - in_off[(size_t)v]);
   if ((rc = dmalloc(&G.h, (size_t)c->n)) || (rc = dmalloc(&G.e, (size_t)c->n)) ||
-      (rc = dmalloc(&G.stamp, (size_t)c->n)) ||
+      (rc = dmalloc(&G.stamp, (size_t)c->n)) || (rc = dmalloc(&G.cycle_base, 1)) ||
       (rc = dmalloc(&G.lists, (size_t)4 * c->n)) || (rc = dmalloc(&G.ctl, (size_t)CTL_SIZE)) ||
       (rc = dmalloc(&G.stats, (size_t)kStatSlot * kStatSpread)) || (rc = dmalloc(&G.src, (size_t)k)) ||
       (rc = dmalloc(&G.snk, (size_t)k)) || (rc = dmalloc(&G.values, (size_t)k)))
@@ -1444,6 +1469,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   ca.stats = G.stats;
   ca.fwd_hops = o.forward_hops < 0 ? 0 : (o.forward_hops > 0 ? o.forward_hops : kDefaultForwardHops);
   ca.stamp = G.stamp;
+  ca.cycle_base = nullptr;
   ca.work = nullptr;
   ca.work_cls = nullptr;
   if (o.record_workload) {  // per-wave busy cycles, summed over this solve's cycle launches
@@ -1472,6 +1498,46 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   const double ratio = o.relabel_time_ratio > 0 ? o.relabel_time_ratio : kDefaultRelabelRatio;
   double last_relabel_ms = 0;
   double cyc_est_ms = 0;  // wall time per cycle launch in the last chunk
+  // captured cycle chunks, one per length (multiples of kGraphPeriod), cached on the graph for
+  // this launch configuration (a recording solve captures its own, released at its end)
+  std::vector<hipGraphExec_t> *gexec = nullptr;
+  struct LocalGraphs {
+    std::vector<hipGraphExec_t> exec;
+    ~LocalGraphs() {
+      for (hipGraphExec_t x : exec)
+        if (x) (void)hipGraphExecDestroy(x);
+    }
+  } local_graphs;
+  if (o.cycle_graphs) {
+    const size_t slots = kGraphChunkMax / kGraphPeriod + 1;
+    if (o.record_workload) {
+      local_graphs.exec.assign(slots, nullptr);
+      gexec = &local_graphs.exec;
+    } else {
+      std::array<int, 9> key = {(int)algo, o.tile_lanes, o.class_lanes[0], o.class_lanes[1],
+                                o.class_lanes[2], o.class_lanes[3], o.class_lanes[4],
+                                o.forward_hops, grid};
+      for (auto &cg : G.cycle_graphs)
+        if (cg.key == key) gexec = &cg.exec;
+      if (!gexec) {
+        G.cycle_graphs.push_back(DevGraph::CycleGraphs{key, std::vector<hipGraphExec_t>(slots, nullptr)});
+        gexec = &G.cycle_graphs.back().exec;
+      }
+    }
+  }
+  int64_t graph_launches = 0;
+  hipEvent_t e_graph[2] = {nullptr, nullptr};
+  struct EventPair {
+    hipEvent_t *e;
+    ~EventPair() {
+      for (int q = 0; q < 2; ++q)
+        if (e[q]) (void)hipEventDestroy(e[q]);
+    }
+  } e_graph_guard{e_graph};
+  if (o.cycle_graphs && o.profile_kernels > 0) {
+    HIP_TRY(hipEventCreate(&e_graph[0]));
+    HIP_TRY(hipEventCreate(&e_graph[1]));
+  }
   const int64_t max_rounds = o.max_rounds > 0 ? o.max_rounds : (int64_t)100000000;
   const int vgrid = std::max(1, std::min((n + 255) / 256, 16 * ctx->num_cu));
   const int kgrid = std::max(1, std::min((G.k + 255) / 256, 16 * ctx->num_cu));
@@ -1608,8 +1674,41 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       chunk = std::min(chunk, K - done);
       const int half = nchunk & 1;
       const double tk0 = now_ms();
-      int timed = 0;
-      for (int q = done; q < done + chunk; ++q) {
+      int timed = 0, graph_chunk = 0;
+      // graph mode: the chunk rounded up to whole periods of the list/counter rotation, replayed
+      // from a captured graph (captured at its first use in this solve)
+      const int gchunk = std::min(kGraphChunkMax, (chunk + kGraphPeriod - 1) / kGraphPeriod * kGraphPeriod);
+      if (o.cycle_graphs && done % kGraphPeriod == 0 && gchunk <= K - done) {
+        hipGraphExec_t &x = (*gexec)[(size_t)(gchunk / kGraphPeriod)];
+        if (!x) {
+          hipGraph_t graph = nullptr;
+          HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+          for (int q = 0; q < gchunk; ++q) {
+            CycleArgs cg = ca;
+            cg.in = sets[2 + (q & 1)];
+            cg.in.cnt = ccnt[q % 3];
+            cg.out = sets[2 + ((q + 1) & 1)];
+            cg.out.cnt = ccnt[(q + 1) % 3];
+            cg.clear_cnt = ccnt[(q + 2) % 3];
+            cg.cycle = q;
+            cg.cycle_base = G.cycle_base;
+            k_cycle<L><<<grid, kBlock, 0, st>>>(lay, cg, disp);
+          }
+          const hipError_t cap_err = hipGetLastError();
+          HIP_TRY(hipStreamEndCapture(st, &graph));
+          HIP_TRY(cap_err);
+          const hipError_t inst_err = hipGraphInstantiate(&x, graph, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(graph);
+          HIP_TRY(inst_err);
+        }
+        k_set_int<<<1, 1, 0, st>>>(G.cycle_base, (int)(cycles + done));
+        if (o.profile_kernels > 0) HIP_TRY(hipEventRecord(e_graph[0], st));  // whole chunk
+        HIP_TRY(hipGraphLaunch(x, st));
+        if (o.profile_kernels > 0) HIP_TRY(hipEventRecord(e_graph[1], st));
+        chunk = gchunk;
+        graph_chunk = gchunk;
+      }
+      for (int q = done; q < done + chunk && !graph_chunk; ++q) {
         ca.in = sets[2 + (q & 1)];
         ca.in.cnt = ccnt[q % 3];
         ca.out = sets[2 + ((q + 1) & 1)];
@@ -1634,6 +1733,13 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       ++nchunk;
       HIP_TRY(hipStreamSynchronize(st));
       cyc_est_ms = (now_ms() - tk0) / chunk;
+      if (graph_chunk && o.profile_kernels > 0) {  // replay time over its launches
+        float gms = 0;
+        HIP_TRY(hipEventElapsedTime(&gms, e_graph[0], e_graph[1]));
+        prof_ms += gms;
+        prof_launches += graph_chunk;
+      }
+      graph_launches += graph_chunk;
       int64_t left = 0;
       for (int c = 0; c < cfg.nclass; ++c) left += hstate[ctl_cnt(CTL_CCNT, done % 3, c)];
       if (left == 0) break;
@@ -1676,6 +1782,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
                                            res->bfs_slots, (int64_t)hs[ST_BU_VERTS],
                                            (int64_t)hs[ST_BU_SLOTS]);
     res->bfs_bottom_up_levels = (int32_t)bfs_bu_levels;
+    res->graph_launches = graph_launches;
     res->profiled_kernel_ms = prof_ms;
     res->profiled_launches = prof_launches;
     res->bfs_levels = bfs_levels;
@@ -2157,7 +2264,7 @@ mf_status mf_rcsr_push_relabel(mf_context *ctx, int algo_type, int V, int E, int
       (st = dmalloc(&G.bcf, (size_t)E)) || (st = dmalloc(&G.roff, (size_t)V + 1)) ||
       (st = dmalloc(&G.rdi, (size_t)E)) || (st = dmalloc(&G.cap0, (size_t)E)) ||
       (st = dmalloc(&G.h, (size_t)V)) || (st = dmalloc(&G.e, (size_t)V)) ||
-      (st = dmalloc(&G.stamp, (size_t)V)) ||
+      (st = dmalloc(&G.stamp, (size_t)V)) || (st = dmalloc(&G.cycle_base, 1)) ||
       (st = dmalloc(&G.lists, (size_t)4 * V)) || (st = dmalloc(&G.ctl, (size_t)CTL_SIZE)) ||
       (st = dmalloc(&G.stats, (size_t)kStatSlot * kStatSpread)) || (st = dmalloc(&G.src, (size_t)1)) ||
       (st = dmalloc(&G.snk, (size_t)1)) || (st = dmalloc(&G.values, (size_t)1)))
