@@ -446,7 +446,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="default launch parameters")
-    ap.add_argument("--tune-sample", type=int, default=256, help="batched: autotune sample")
+    ap.add_argument("--tune-sample", type=int, default=1024,
+                    help="batched: autotune sample (enough instances to fill every workgroup)")
     ap.add_argument("--instances", type=int, default=4096, help="batched: number of instances")
     ap.add_argument("--one-by-one", type=int, default=256, help="batched: one-by-one sample size")
     ap.add_argument("--cpu-sample", type=int, default=64, help="batched: CPU baseline sample")

@@ -154,8 +154,10 @@ struct LayoutRCSR {
 // logic error can never write outside the allocation.
 struct ListSet {
   int *items;
-  int cls_off[kNumClasses];
-  int cls_cap[kNumClasses];
+  // region offset of class c at cls[c], its capacity at cls[kNumClasses + c]; the table lives in
+  // device memory (or LDS), not in the descriptor: a descriptor copied to a local variable and
+  // indexed by class at run time would otherwise be placed in scratch memory
+  const int *cls;
   int *cnt;     // kNumClasses counters, cnt[c * cstride]
   int cstride;  // kCtlStride for counters in device memory, 1 for counters in LDS
   int *ovf;
@@ -179,7 +181,7 @@ __device__ __forceinline__ V pick(const V (&a)[N], int i) {
 // earlier kernel or, inside the one-workgroup BFS, lives in LDS behind a workgroup barrier.
 __device__ __forceinline__ int list_count(const ListSet &L, int c) {
   const int k = L.cnt[c * L.cstride];
-  const int cap = pick(L.cls_cap, c);
+  const int cap = L.cls[kNumClasses + c];
   return k < cap ? k : cap;
 }
 
@@ -207,7 +209,7 @@ __device__ __forceinline__ void append(const ListSet &L, int item, bool want, in
     if (lane == leader) base = atomicAdd(&L.cnt[c * L.cstride], __popcll(mask));
     base = __shfl(base, leader);
     if (mine) {
-      if (base + rank < L.cls_cap[c]) L.items[L.cls_off[c] + base + rank] = item;
+      if (base + rank < L.cls[kNumClasses + c]) L.items[L.cls[c] + base + rank] = item;
       else *L.ovf = 1;
     }
   }
@@ -258,7 +260,7 @@ __device__ __forceinline__ void staged_append(Stage &S, const ListSet &L, int it
     if (lane == sl) gbase = atomicAdd(&L.cnt[c * L.cstride], __popcll(spill));
     gbase = __shfl(gbase, sl);
     if (mine && !fits) {
-      if (gbase + srank < pick(L.cls_cap, c)) L.items[pick(L.cls_off, c) + gbase + srank] = item;
+      if (gbase + srank < L.cls[kNumClasses + c]) L.items[L.cls[c] + gbase + srank] = item;
       else *L.ovf = 1;
     }
   }
@@ -277,7 +279,7 @@ __device__ __forceinline__ void stage_flush(Stage &S, const ListSet &L, int ncla
   for (int c = 0; c < kNumClasses; ++c) {
     if (c >= nclass) break;
     const int k = min(S.cnt[c], kStageCap);
-    const int base = S.base[c], cap = pick(L.cls_cap, c), off = pick(L.cls_off, c);
+    const int base = S.base[c], cap = L.cls[kNumClasses + c], off = L.cls[c];
     for (int i = (int)threadIdx.x; i < k; i += (int)blockDim.x) {
       if (base + i < cap) L.items[off + base + i] = S.items[c][i];
       else *L.ovf = 1;

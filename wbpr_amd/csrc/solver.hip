@@ -307,7 +307,7 @@ __device__ __forceinline__ bool claim_queued(const CycleArgs &a, int u, int lane
 template <int T, class L>
 __device__ __forceinline__ void cycle_class(const L &g, const CycleArgs &a, Stage *S, int c,
                                             int cnt, int b, int nblk, Stats &st) {
-  const int base = pick(a.in.cls_off, c);
+  const int base = a.in.cls[c];
   if constexpr (T <= 64) {
     constexpr int tpb = kBlock / T;
     const int tile = b * tpb + (int)threadIdx.x / T;
@@ -393,10 +393,11 @@ struct BfsStages {
   Stage *a;
 };
 
+// cw, aw: the degree classes of w in the frontier lists and the cycle worklists (read by the
+// caller, ahead of time where latency matters)
 __device__ __forceinline__ void bfs_append(const BfsStages &S, const BfsArgs &a,
-                                           const ListSet &out, int w, bool claim, bool act) {
-  const int cw = (claim && a.vcls) ? a.vcls[w] : 0;
-  const int aw = (act && a.acls) ? a.acls[w] : 0;
+                                           const ListSet &out, int w, bool claim, bool act, int cw,
+                                           int aw) {
   if (S.f) {
     staged_append(*S.f, out, w, claim, cw, a.nclass);
     staged_append(*S.a, a.active, w, act, aw, a.nclass_act);
@@ -406,13 +407,24 @@ __device__ __forceinline__ void bfs_append(const BfsStages &S, const BfsArgs &a,
   }
 }
 
-// Slots one BFS lane keeps in flight: the four dependent steps (slot record, reverse residual,
-// neighbour height, claim) are each issued for the whole batch before waiting on the next.
+__device__ __forceinline__ void bfs_append(const BfsStages &S, const BfsArgs &a,
+                                           const ListSet &out, int w, bool claim, bool act) {
+  const int cw = (claim && a.vcls) ? a.vcls[w] : 0;
+  const int aw = (act && a.acls) ? a.acls[w] : 0;
+  bfs_append(S, a, out, w, claim, act, cw, aw);
+}
+
+// Slots one BFS lane keeps in flight.  Grid-wide levels (large frontiers, bandwidth matters)
+// issue the dependent steps -- slot record, reverse residual, neighbour height, claim, class --
+// one after the other for the whole batch; the one-workgroup kernel of a single graph (small
+// frontiers, thousands of levels, latency matters) reads the neighbour's height, excess and
+// classes together with the reverse residual, ahead of knowing whether the arc has residual
+// capacity.  Excesses do not change during a global relabel.
 constexpr int kBfsBatch = 4;
 
 // Lanes lane, lane + step, ... of a group of `step` lanes (a tile, a block or a group of
 // blocks: nothing here is shared inside the group) expand u's slots.
-template <class L>
+template <bool ahead, class L>
 __device__ __forceinline__ void bfs_vertex(const L &g, const BfsArgs &a, const ListSet &out,
                                            const BfsStages &S, int u, int lane, int step,
                                            int level, long long &ns) {
@@ -422,7 +434,7 @@ __device__ __forceinline__ void bfs_vertex(const L &g, const BfsArgs &a, const L
     const int2 r = g.range(u, seg);
     if (lane == 0) ns += r.y - r.x;
     for (int base = r.x + lane; base < r.y; base += step * B) {
-      int w[B], idx[B], rin[B], hw[B];
+      int w[B], idx[B], rin[B], hw[B], ew[B], cw[B], aw[B];
       bool claim[B];
 #pragma unroll
       for (int b = 0; b < B; ++b) {
@@ -430,35 +442,57 @@ __device__ __forceinline__ void bfs_vertex(const L &g, const BfsArgs &a, const L
         idx[b] = 0;
         if (base + b * step < r.y) g.bfs_slot(seg, base + b * step, w[b], idx[b]);
       }
+      if constexpr (ahead) {
 #pragma unroll
-      for (int b = 0; b < B; ++b) rin[b] = (w[b] >= 0 && w[b] != u) ? g.bfs_rin(seg, idx[b]) : 0;
+        for (int b = 0; b < B; ++b) {
+          const bool ok = w[b] >= 0 && w[b] != u;
+          rin[b] = ok ? g.bfs_rin(seg, idx[b]) : 0;
+          hw[b] = ok ? a.h[w[b]] : -1;
+          ew[b] = ok ? a.e[w[b]] : 0;
+          cw[b] = (ok && a.vcls) ? a.vcls[w[b]] : 0;
+          aw[b] = (ok && a.acls) ? a.acls[w[b]] : 0;
+        }
+      } else {
 #pragma unroll
-      for (int b = 0; b < B; ++b) hw[b] = rin[b] > 0 ? a.h[w[b]] : -1;
+        for (int b = 0; b < B; ++b) rin[b] = (w[b] >= 0 && w[b] != u) ? g.bfs_rin(seg, idx[b]) : 0;
+#pragma unroll
+        for (int b = 0; b < B; ++b) hw[b] = rin[b] > 0 ? a.h[w[b]] : -1;
+      }
 #pragma unroll
       for (int b = 0; b < B; ++b) {
         claim[b] = false;
+        if (rin[b] <= 0) continue;
         if (hw[b] == a.H + 1) a.flags[0] = 1;  // residual path from a source: invariant broken
         else if (hw[b] == a.H) claim[b] = atomicCAS(&a.h[w[b]], a.H, level + 1) == a.H;
       }
+      if constexpr (ahead) {
 #pragma unroll
-      for (int b = 0; b < B; ++b) bfs_append(S, a, out, w[b], claim[b], claim[b] && a.e[w[b]] > 0);
+        for (int b = 0; b < B; ++b)
+          bfs_append(S, a, out, w[b], claim[b], claim[b] && ew[b] > 0, cw[b], aw[b]);
+      } else {
+#pragma unroll
+        for (int b = 0; b < B; ++b) bfs_append(S, a, out, w[b], claim[b], claim[b] && a.e[w[b]] > 0);
+      }
     }
   }
 }
 
-template <int T, class L>
+template <int T, bool ahead, class L>
 __device__ __forceinline__ void bfs_class(const L &g, const BfsArgs &a, const ListSet &in,
                                           const ListSet &out, const BfsStages &S, int c, int cnt,
                                           int tile, int ntiles, int lane, int level, long long &nv,
                                           long long &ns) {
-  const int base = pick(in.cls_off, c);
+  const int base = in.cls[c];
   for (int i = tile; i < cnt; i += ntiles) {
-    bfs_vertex(g, a, out, S, in.items[base + i], lane, T, level, ns);
+    bfs_vertex<ahead>(g, a, out, S, in.items[base + i], lane, T, level, ns);
     if (lane == 0) ++nv;
   }
 }
 
-template <class L>
+// ahead: read neighbour state ahead of the residual check (the latency-bound one-workgroup BFS
+// of a single graph); grid-wide levels and the batched kernel, whose thousand workgroups share
+// the memory system, read it only for arcs with residual capacity.
+template <bool ahead, class L>
 __device__ __forceinline__ void bfs_dispatch(const L &g, const BfsArgs &a, const ListSet &in,
                                              const ListSet &out, const BfsStages &S, int c,
                                              int lanes, int cnt, int tid, int nthreads, int level,
@@ -469,7 +503,7 @@ __device__ __forceinline__ void bfs_dispatch(const L &g, const BfsArgs &a, const
   switch (lanes) {
 #define MF_BFS_CASE(LANES, TT)                                                                \
   case LANES:                                                                                 \
-    bfs_class<TT>(g, a, in, out, S, c, cnt, tid / TT, nthreads / TT, tid % TT, level, nv, ns); \
+    bfs_class<TT, ahead>(g, a, in, out, S, c, cnt, tid / TT, nthreads / TT, tid % TT, level, nv, ns); \
     break;
     MF_BFS_CASE(1, 1)
     MF_BFS_CASE(8, 2)
@@ -482,9 +516,9 @@ __device__ __forceinline__ void bfs_dispatch(const L &g, const BfsArgs &a, const
       // the frontier (one hub of 200,000 slots would keep a single tile busy for milliseconds)
       const int per = (int)blockDim.x;
       const BlockGroups bg = block_groups(tid / per, nthreads / per, cnt);
-      const int base = pick(in.cls_off, c);
+      const int base = in.cls[c];
       for (int i = bg.group; i < cnt; i += bg.ngroups) {
-        bfs_vertex(g, a, out, S, in.items[base + i], bg.lane, bg.width, level, ns);
+        bfs_vertex<ahead>(g, a, out, S, in.items[base + i], bg.lane, bg.width, level, ns);
         if (bg.lane == 0) ++nv;
       }
       break;
@@ -532,13 +566,16 @@ __global__ __launch_bounds__(kBlock) void k_bfs_level(L g, BfsArgs a, Dispatch d
   const int b0 = pick(d.blk_begin, c);
   const int nblk = pick(d.blk_begin, c + 1) - b0;
   const int b = (int)blockIdx.x - b0;
-  ListSet in = (level & 1) ? a.fr[1] : a.fr[0];
+  // the two frontier lists share their class regions; only the item arrays alternate (selecting
+  // whole descriptors would copy them to scratch memory)
+  ListSet in = a.fr[0], out = a.fr[0];
+  in.items = (level & 1) ? a.fr[1].items : a.fr[0].items;
+  out.items = (level & 1) ? a.fr[0].items : a.fr[1].items;
   in.cnt = pick(a.fcnt, level % 3);
-  ListSet out = (level & 1) ? a.fr[0] : a.fr[1];
   out.cnt = pick(a.fcnt, (level + 1) % 3);
   const int cnt = list_count(in, c);
   long long nv = 0, ns = 0;
-  bfs_dispatch(g, a, in, out, BfsStages{&sf, &sa}, c, pick(d.lanes, c), cnt,
+  bfs_dispatch<false>(g, a, in, out, BfsStages{&sf, &sa}, c, pick(d.lanes, c), cnt,
                b * kBlock + (int)threadIdx.x, nblk * kBlock, level, nv, ns);
   stage_flush(sf, out, a.nclass);
   stage_flush(sa, a.active, a.nclass_act);
@@ -635,7 +672,8 @@ __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(L g, BfsArgs a, Dispat
   const int nblk = pick(d.blk_begin, c + 1) - b0;
   const int tid = ((int)blockIdx.x - b0) * kBlock + (int)threadIdx.x;
   const int nthreads = nblk * kBlock;
-  ListSet out = (level & 1) ? a.fr[0] : a.fr[1];
+  ListSet out = a.fr[0];  // the frontier lists differ only in their item arrays
+  out.items = (level & 1) ? a.fr[0].items : a.fr[1].items;
   out.cnt = pick(a.fcnt, (level + 1) % 3);
   long long nv = 0, ns = 0;
   switch (pick(d.lanes, c)) {  // same lanes per vertex as the top-down levels
@@ -691,16 +729,17 @@ __global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d,
     __syncthreads();
     const int size = s_size;
     if (size == 0 || size > fmax || s_big) break;
-    ListSet in = (level & 1) ? a.fr[1] : a.fr[0];
+    ListSet in = a.fr[0], out = a.fr[0];  // same class regions, alternating item arrays
+    in.items = (level & 1) ? a.fr[1].items : a.fr[0].items;
+    out.items = (level & 1) ? a.fr[0].items : a.fr[1].items;
     in.cnt = s_cnt[level % 3];
     in.cstride = 1;
-    ListSet out = (level & 1) ? a.fr[0] : a.fr[1];
     out.cnt = s_cnt[(level + 1) % 3];
     out.cstride = 1;
     for (int c = 0; c < d.nclass; ++c) {
       const int cnt = list_count(in, c);
       if (cnt == 0) continue;
-      bfs_dispatch(g, a, in, out, BfsStages{nullptr, nullptr}, c, pick(d.lanes, c), cnt, tid,
+      bfs_dispatch<true>(g, a, in, out, BfsStages{nullptr, nullptr}, c, pick(d.lanes, c), cnt, tid,
                    kWgBlock, level, nv, ns);
     }
     __syncthreads();
@@ -811,7 +850,7 @@ struct BatchArgs {
 template <class L>
 __device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &ca, int c, int lanes,
                                                   int cnt, Stats &st) {
-  const int base = pick(ca.in.cls_off, c);
+  const int base = ca.in.cls[c];
   const int tid = (int)threadIdx.x;
   switch (lanes) {
 #define MF_BATCH_CASE(TT)                                                          \
@@ -840,24 +879,29 @@ __device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &c
 template <class L>
 __device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, int inst,
                                int (*s_f)[kNumClasses], int (*s_c)[kNumClasses], int *s_flag,
-                               int *s_dec, Stats &st, long long &nv, long long &ns) {
+                               int *s_dec, int (*s_tab)[2 * kNumClasses], Stats &st,
+                               long long &nv, long long &ns) {
   const int tid = (int)threadIdx.x;
   const int vb = b.vbase[inst], n = b.nver[inst], s = b.src[inst], t = b.snk[inst];
   const int H = n;  // heights are distances inside the instance
-  // frontier lists F0, F1 and cycle worklists S0, S1, each split into class regions; kept as
-  // four named values (an indexed local array would live in scratch memory)
-  auto make_set = [&](int q) {
-    ListSet ls;
-    const int *pp = (q < 2 ? b.bpop : b.pop) + (size_t)inst * kNumClasses;
-    const int nc = q < 2 ? kNumClasses : b.nclass;
-    ls.items = b.lists + (size_t)q * b.N;
+  // class regions of the instance's frontier lists (table 0) and cycle worklists (table 1) in
+  // the packed list arrays, in LDS
+  if (tid < 2) {
+    const int *pp = (tid == 0 ? b.bpop : b.pop) + (size_t)inst * kNumClasses;
+    const int nc = tid == 0 ? kNumClasses : b.nclass;
     int acc = vb;
-#pragma unroll
     for (int c = 0; c < kNumClasses; ++c) {
-      ls.cls_off[c] = acc;
-      ls.cls_cap[c] = c < nc ? pp[c] : 0;
+      s_tab[tid][c] = acc;
+      s_tab[tid][kNumClasses + c] = c < nc ? pp[c] : 0;
       if (c < nc) acc += pp[c];
     }
+  }
+  __syncthreads();
+  // frontier lists F0, F1 and cycle worklists S0, S1
+  auto make_set = [&](int q) {
+    ListSet ls;
+    ls.items = b.lists + (size_t)q * b.N;
+    ls.cls = s_tab[q < 2 ? 0 : 1];
     ls.cnt = nullptr;
     ls.cstride = 1;  // the workgroup's counters live in LDS
     ls.ovf = b.ovf;
@@ -952,7 +996,7 @@ __device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, i
       for (int c = 0; c < kNumClasses; ++c) {
         const int cnt = list_count(in, c);
         if (cnt)
-          bfs_dispatch(g, ba, in, out, BfsStages{nullptr, nullptr}, c, pick(b.blanes, c), cnt,
+          bfs_dispatch<false>(g, ba, in, out, BfsStages{nullptr, nullptr}, c, pick(b.blanes, c), cnt,
                        tid, kBlock, level, nv, ns);
       }
       __syncthreads();
@@ -1005,6 +1049,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_wg(L g, BatchArgs b) {
   __shared__ int s_flag[2];
   __shared__ int s_dec;
   __shared__ int s_inst;
+  __shared__ int s_tab[2][2 * kNumClasses];
   Stats st{0, 0, 0, 0};
   long long nv = 0, ns = 0;
   for (;;) {
@@ -1013,7 +1058,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_wg(L g, BatchArgs b) {
     const int i = s_inst;
     __syncthreads();
     if (i >= b.k) break;
-    batch_instance(g, b, b.order[i], s_f, s_c, s_flag, &s_dec, st, nv, ns);
+    batch_instance(g, b, b.order[i], s_f, s_c, s_flag, &s_dec, s_tab, st, nv, ns);
   }
   block_flush(st.active, st.slots, st.pushes, st.relabels, b.stats, ST_ACTIVE);
   block_flush(nv, ns, 0, 0, b.stats, ST_BFS_VERTS);
@@ -1183,6 +1228,7 @@ struct DevGraph {
   unsigned char *work_cls = nullptr;   //   and the degree class of each wave's block
   int64_t work_waves = 0;
   int *cycle_base = nullptr;  // graph replays: first cycle number of the replayed chunk
+  int *cls_tab = nullptr;     // class tables of the frontier lists and the cycle worklists
   // captured cycle chunks, kept across solves: one set per configuration of the launches
   struct CycleGraphs {
     std::array<int, 9> key;
@@ -1235,7 +1281,7 @@ static void free_dev(DevGraph *g) {
   void *ptrs[] = {g->off, g->dc,  g->rev, g->dst,    g->cr,    g->fdc,   g->bcf,
                   g->roff, g->rdi, g->cap0, g->src,   g->snk,   g->values, g->h,
                   g->e,   g->lists, g->ctl, g->stats, g->vcls, g->vorder, g->stamp, g->fslot,
-                  g->work, g->work_cls, g->cycle_base};
+                  g->work, g->work_cls, g->cycle_base, g->cls_tab};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -1381,6 +1427,7 @@ static mf_status upload(mf_context *ctx, const mf_csr *c, const int32_t *src, co
                         (in_off[(size_t)v + 1] - in_off[(size_t)v]);
   if ((rc = dmalloc(&G.h, (size_t)c->n)) || (rc = dmalloc(&G.e, (size_t)c->n)) ||
       (rc = dmalloc(&G.stamp, (size_t)c->n)) || (rc = dmalloc(&G.cycle_base, 1)) ||
+      (rc = dmalloc(&G.cls_tab, (size_t)4 * kNumClasses)) ||
       (rc = dmalloc(&G.lists, (size_t)4 * c->n)) || (rc = dmalloc(&G.ctl, (size_t)CTL_SIZE)) ||
       (rc = dmalloc(&G.stats, (size_t)kStatSlot * kStatSpread)) || (rc = dmalloc(&G.src, (size_t)k)) ||
       (rc = dmalloc(&G.snk, (size_t)k)) || (rc = dmalloc(&G.values, (size_t)k)))
@@ -1419,17 +1466,24 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   const bool bu_always = o.bfs_direction == 2;
 
   // list sets: F0, F1 (BFS frontier), S0, S1 (cycle worklists); class regions by population
+  // class tables: 0 for the frontier lists, 1 for the cycle worklists (the lists of a pair
+  // share their class regions)
+  int tab[2][2 * kNumClasses];
+  for (int q = 0; q < 2; ++q) {
+    const int nc = q == 0 ? bcfg.nclass : cfg.nclass;
+    const int64_t *pp = q == 0 ? G.vc_pop : pop;
+    int64_t acc = 0;
+    for (int c = 0; c < kNumClasses; ++c) {
+      tab[q][c] = (int)acc;
+      tab[q][kNumClasses + c] = c < nc ? (int)pp[c] : 0;
+      if (c < nc) acc += pp[c];
+    }
+  }
+  HIP_TRY(hipMemcpy(G.cls_tab, tab, sizeof(tab), hipMemcpyHostToDevice));
   ListSet sets[4];
   for (int q = 0; q < 4; ++q) {
     sets[q].items = G.lists + (size_t)q * n;
-    const int nc = q < 2 ? bcfg.nclass : cfg.nclass;
-    const int64_t *pp = q < 2 ? G.vc_pop : pop;
-    int64_t acc = 0;
-    for (int c = 0; c < kNumClasses; ++c) {
-      sets[q].cls_off[c] = (int)acc;
-      sets[q].cls_cap[c] = c < nc ? (int)pp[c] : 0;
-      if (c < nc) acc += pp[c];
-    }
+    sets[q].cls = G.cls_tab + (q < 2 ? 0 : 2 * kNumClasses);
     sets[q].cnt = nullptr;
     sets[q].cstride = kCtlStride;
     sets[q].ovf = G.ctl + CTL_FLAGS + 1;
@@ -2265,6 +2319,7 @@ mf_status mf_rcsr_push_relabel(mf_context *ctx, int algo_type, int V, int E, int
       (st = dmalloc(&G.rdi, (size_t)E)) || (st = dmalloc(&G.cap0, (size_t)E)) ||
       (st = dmalloc(&G.h, (size_t)V)) || (st = dmalloc(&G.e, (size_t)V)) ||
       (st = dmalloc(&G.stamp, (size_t)V)) || (st = dmalloc(&G.cycle_base, 1)) ||
+      (st = dmalloc(&G.cls_tab, (size_t)4 * kNumClasses)) ||
       (st = dmalloc(&G.lists, (size_t)4 * V)) || (st = dmalloc(&G.ctl, (size_t)CTL_SIZE)) ||
       (st = dmalloc(&G.stats, (size_t)kStatSlot * kStatSpread)) || (st = dmalloc(&G.src, (size_t)1)) ||
       (st = dmalloc(&G.snk, (size_t)1)) || (st = dmalloc(&G.values, (size_t)1)))
