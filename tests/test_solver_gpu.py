@@ -156,6 +156,66 @@ def test_rmat18_hub_vertices(mf, orc, ctx, layout):
     solve_and_certify(mf, orc, ctx, g, "vc", layout, orc.dinic_maxflow(g))
 
 
+def giant_graph(mf, seed):
+    """Giants of 20K..200K slots (split into parts by the cycle kernel): the source feeds them,
+    they fan out to a middle layer that drains through a second layer to the sink, they are
+    linked to each other in both directions, and middle vertices point back into some of them
+    (in-arcs: RCSR parts then cross from the forward into the reverse segment)."""
+    rng = np.random.default_rng(seed)
+    sizes = [20000, 45000, 70000, 130000, 200000]
+    n_mid, n_two = 220000, 4000
+    s, t, g0 = 0, 1, 2
+    mid0 = g0 + len(sizes)
+    two0 = mid0 + n_mid
+    n = two0 + n_two
+    u, v, c = [], [], []
+
+    def arcs(a, b, cap):
+        u.append(np.asarray(a, dtype=np.int64))
+        v.append(np.asarray(b, dtype=np.int64))
+        c.append(np.asarray(cap, dtype=np.int64))
+
+    for k, d in enumerate(sizes):
+        gv = g0 + k
+        out = rng.choice(n_mid, size=d, replace=False) + mid0
+        caps = rng.integers(1, 20, size=d)
+        arcs(np.full(d, gv), out, caps)
+        arcs([s], [gv], [int(caps.sum() * rng.uniform(0.3, 1.2))])
+        if k % 2:  # in-arcs from the middle layer
+            back = rng.choice(n_mid, size=d // 3, replace=False) + mid0
+            arcs(back, np.full(back.size, gv), rng.integers(1, 10, size=back.size))
+        for k2 in range(len(sizes)):
+            if k2 != k:
+                arcs([gv], [g0 + k2], [int(rng.integers(1, 5000))])
+    mids = np.arange(n_mid) + mid0
+    for _ in range(2):
+        arcs(mids, rng.integers(0, n_two, size=n_mid) + two0, rng.integers(1, 30, size=n_mid))
+    arcs(np.arange(n_two) + two0, np.full(n_two, t), rng.integers(1, 400, size=n_two))
+    return mf.Graph.from_arcs(n, np.concatenate(u), np.concatenate(v), np.concatenate(c), s, t)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_giant_vertices(mf, orc, ctx, layout):
+    # vertices above 16,384 slots are discharged in parts of 4,096 slots by separate blocks that
+    # meet only through atomics; parity with Dinic, both certificates and the cut on every layout
+    # (VC: thread-centric TC scans a 200,000-slot vertex with one lane, minutes per solve)
+    for seed in (1, 2):
+        g = giant_graph(mf, seed)
+        deg = np.diff(g.offsets) + np.bincount(g.dst, minlength=g.n)
+        assert (deg > 16384).sum() == 5 and deg.max() > 200000
+        r = solve_and_certify(mf, orc, ctx, g, "vc", layout, orc.dinic_maxflow(g))
+        assert r.pushes > 0
+
+
+def test_giant_vertices_fixed_cadence(mf, orc, ctx):
+    # many cycles per relabel: giants relabel and push over many consecutive cycles, so every
+    # part's state must be reset by the last part of each cycle
+    g = giant_graph(mf, 3)
+    expect = orc.dinic_maxflow(g)
+    for k in (1, 3, 50):
+        solve_and_certify(mf, orc, ctx, g, "vc", "bcsr", expect, cycles_per_relabel=k)
+
+
 def test_rmat22_full_size(mf, orc, ctx, golden):
     g, value = generated_graph(mf, golden, "rmat22_s1")
     solve_and_certify(mf, orc, ctx, g, "vc", "bcsr", value)
@@ -334,7 +394,7 @@ def test_workload_record(mf, ctx, golden):
         assert r.value == value
         cyc, cls = dg.read_workload()
         assert len(cyc) > 0 and len(cyc) % 4 == 0 and cyc.sum() > 0
-        assert cls.max() < 5
+        assert cls.max() < 6
         r2 = dg.solve(mf.ALGO_VC)  # without recording the last record stays readable
         assert r2.value == value and len(dg.read_workload()[0]) == len(cyc)
     finally:

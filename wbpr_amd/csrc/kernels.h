@@ -13,8 +13,15 @@
 
 namespace mfk {
 
-constexpr int kNumClasses = 5;   // degree classes of the worklists
+constexpr int kNumClasses = 6;   // degree classes of the worklists
 constexpr int kBlock = 256;      // threads per block of the cycle / BFS-level kernels
+// Giant vertices (more than kGiantSlots residual slots) form the last degree class.  A cycle
+// splits each giant into parts of kPartSlots slots, one block per part: the largest hubs of a
+// power-law graph hold hundreds of thousands of slots, and a single block scanning one would
+// run alone for most of the launch.  kSplitLanes marks the split class in the lanes tables.
+constexpr int kGiantSlots = 16384;
+constexpr int kPartSlots = 4096;
+constexpr int kSplitLanes = 4096;
 constexpr int kWgBlock = 1024;   // threads of the one-workgroup BFS kernel
 constexpr int kMultiPushLanes = 64;  // tiles this wide (and whole blocks) push along many arcs
 
@@ -44,6 +51,7 @@ struct LayoutBCSR {
   }
   __device__ __forceinline__ int target(int key) const { return dc[key].x; }
   __device__ __forceinline__ int residual(int key) const { return dc[key].y; }
+  __device__ __forceinline__ int *res_ptr(int key) const { return &dc[key].y; }
   __device__ __forceinline__ void push(int key, int d) const {
     atomicSub(&dc[key].y, d);
     atomicAdd(&dc[rev[key]].y, d);
@@ -73,6 +81,7 @@ struct LayoutBCSR2 {
   }
   __device__ __forceinline__ int target(int key) const { return dst[key]; }
   __device__ __forceinline__ int residual(int key) const { return cr[key].x; }
+  __device__ __forceinline__ int *res_ptr(int key) const { return &cr[key].x; }
   __device__ __forceinline__ void push(int key, int d) const {
     const int rv = cr[key].y;
     atomicSub(&cr[key].x, d);
@@ -118,6 +127,9 @@ struct LayoutRCSR {
   }
   __device__ __forceinline__ int residual(int key) const {
     return key < m ? fdc[key].y : bcf[rdi[key - m].y];
+  }
+  __device__ __forceinline__ int *res_ptr(int key) const {
+    return key < m ? &fdc[key].y : &bcf[rdi[key - m].y];
   }
   __device__ __forceinline__ void push(int key, int d) const {
     if (key < m) {

@@ -8,11 +8,14 @@
 namespace mf {
 
 // Degree classes of the worklists: class k holds vertices with slot degree <= maxdeg[k] and is
-// processed with lanes[k] lanes per vertex (1 = thread-centric, 256 = whole block).
+// processed with lanes[k] lanes per vertex (1 = thread-centric, 256 = whole block, the last VC
+// class = giants split over several blocks).  mf_options.class_lanes sets the first kTunedClasses.
+constexpr int kClasses = 6;
+constexpr int kTunedClasses = 5;
 struct ClassConfig {
   int nclass = 1;
-  int lanes[5] = {1, 0, 0, 0, 0};
-  int maxdeg[5] = {0, 0, 0, 0, 0};
+  int lanes[kClasses] = {1, 0, 0, 0, 0, 0};
+  int64_t maxdeg[kClasses] = {0, 0, 0, 0, 0, 0};
 };
 
 ClassConfig class_config(mf_algo algo, int tile_lanes, const uint16_t *class_lanes = nullptr);

@@ -59,6 +59,10 @@ struct CycleArgs {
   const int *cycle_base;  // graph replays: the chunk's first cycle number, set before each replay
   unsigned long long *work;  // optional: busy clock cycles of every wave, summed over launches
   unsigned char *work_cls;   // optional: degree class of every wave's block
+  // giants, per position in the cycle's giant list: the lowest (height, key) any part found and
+  // the parts that have finished (both back to ~0 / 0 once the last part is done)
+  unsigned long long *gbest;
+  int *garrive;
 };
 
 // This cycle's number; a launch replayed from a captured graph adds the chunk's first cycle
@@ -82,9 +86,31 @@ struct Cand {
   }
 };
 
-// Lane `first` of a group of `step` lanes scans slots first, first + step, ... of u, B slots in
-// flight: the B slot records are loaded together, then the B neighbour heights, so a lane waits
-// for two dependent loads per B slots instead of two per slot.
+// Lane `first` of a group of `step` lanes scans slots begin + first, begin + first + step, ...
+// (up to end) of segment seg of u, B slots in flight: the B slot records are loaded together,
+// then the B neighbour heights, so a lane waits for two dependent loads per B slots instead of
+// two per slot.
+template <int B, class L>
+__device__ __forceinline__ void scan_seg(const L &g, const int *h, int u, int seg, int begin,
+                                         int end, int first, int step, Cand &c) {
+  for (int j0 = begin + first; j0 < end; j0 += B * step) {
+    int w[B], res[B], key[B], hw[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      w[b] = u;
+      res[b] = 0;
+      key[b] = 0;
+      if (j0 + b * step < end) g.scan(seg, j0 + b * step, w[b], res[b], key[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) hw[b] = (res[b] > 0 && w[b] != u) ? h[w[b]] : 0;
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+      if (res[b] > 0 && w[b] != u) c.consider(hw[b], key[b], w[b], res[b]);
+  }
+}
+
+// All slots of u; deg = their number.
 template <int B, class L>
 __device__ __forceinline__ void scan_slots(const L &g, const int *h, int u, int first, int step,
                                            Cand &c, int &deg) {
@@ -93,22 +119,33 @@ __device__ __forceinline__ void scan_slots(const L &g, const int *h, int u, int 
   for (int seg = 0; seg < L::kSeg; ++seg) {
     const int2 r = g.range(u, seg);
     deg += r.y - r.x;
-    for (int j0 = r.x + first; j0 < r.y; j0 += B * step) {
-      int w[B], res[B], key[B], hw[B];
-#pragma unroll
-      for (int b = 0; b < B; ++b) {
-        w[b] = u;
-        res[b] = 0;
-        key[b] = 0;
-        if (j0 + b * step < r.y) g.scan(seg, j0 + b * step, w[b], res[b], key[b]);
-      }
-#pragma unroll
-      for (int b = 0; b < B; ++b) hw[b] = (res[b] > 0 && w[b] != u) ? h[w[b]] : 0;
-#pragma unroll
-      for (int b = 0; b < B; ++b)
-        if (res[b] > 0 && w[b] != u) c.consider(hw[b], key[b], w[b], res[b]);
-    }
+    scan_seg<B>(g, h, u, seg, r.x, r.y, first, step, c);
   }
+}
+
+// Slots [lo, hi) of u, counted over its segments in order.
+template <int B, class L>
+__device__ __forceinline__ void scan_range(const L &g, const int *h, int u, int lo, int hi,
+                                           int first, int step, Cand &c) {
+  int acc = 0;
+#pragma unroll
+  for (int seg = 0; seg < L::kSeg; ++seg) {
+    const int2 r = g.range(u, seg);
+    const int len = r.y - r.x;
+    scan_seg<B>(g, h, u, seg, r.x + max(lo - acc, 0), r.x + min(hi - acc, len), first, step, c);
+    acc += len;
+  }
+}
+
+template <class L>
+__device__ __forceinline__ int slot_degree(const L &g, int u) {
+  int deg = 0;
+#pragma unroll
+  for (int seg = 0; seg < L::kSeg; ++seg) {
+    const int2 r = g.range(u, seg);
+    deg += r.y - r.x;
+  }
+  return deg;
 }
 
 // Slots in flight per lane: a thread-centric lane covers a whole adjacency list alone, a hub
@@ -151,6 +188,26 @@ struct Plan {
 // excess until its own step).  Instead of queueing w it may discharge w itself, now: the
 // returned w (or -1) is what the caller continues with.  fcls: forwarding allowed into vertices
 // of degree class <= fcls (-1: no forwarding), so a hub never lands on a narrow tile.
+// This lane pushes d along its candidate arc; true when the push gave w its first excess (w
+// must then be queued, unless it is a sink).
+template <class L>
+__device__ __forceinline__ bool push_arc(const L &g, const CycleArgs &a, const Cand &c, int d,
+                                         Stats &st) {
+  if (d <= 0) return false;
+  const int key = (int)(unsigned)(c.best & 0xffffffffull);
+  const int old_w = atomicAdd(&a.e[c.w], d);
+  g.push(key, d);
+  st.pushes++;
+  return old_w == 0 && (int)(c.best >> 32) > 0;  // height 0 only at sinks
+}
+
+// Queue item for the next cycle (every lane of the wave calls this).
+__device__ __forceinline__ void queue_next(const CycleArgs &a, Stage *S, int item, bool want) {
+  const int cls = (want && a.vcls) ? a.vcls[item] : 0;
+  if (S) staged_append(*S, a.out, item, want, cls, a.nclass);
+  else append(a.out, item, want, cls, a.nclass);
+}
+
 template <class L>
 __device__ __forceinline__ int finish_vertex(const L &g, const CycleArgs &a, Stage *S, int u,
                                              int hu, const Plan &p, const Cand &mine,
@@ -170,13 +227,7 @@ __device__ __forceinline__ int finish_vertex(const L &g, const CycleArgs &a, Sta
       a.h[u] = p.hnew;
       st.relabels++;
     }
-    if (p.d > 0) {
-      const int key = (int)(unsigned)(mine.best & 0xffffffffull);
-      const int old_w = atomicAdd(&a.e[mine.w], p.d);
-      g.push(key, p.d);
-      want_w = old_w == 0 && (int)(mine.best >> 32) > 0;  // height 0 only at sinks
-      st.pushes++;
-    }
+    want_w = push_arc(g, a, mine, p.d, st);
     if (leader) {
       const int old_u = atomicSub(&a.e[u], p.total);
       want_u = old_u - p.total > 0;
@@ -209,11 +260,8 @@ __device__ __forceinline__ void plan_heights(unsigned long long best, int hu, in
   p.hnew = p.dead ? H : max(hu, hmin + 1);
 }
 
-__device__ __forceinline__ long long admissible_residual(const Plan &p, const Cand &c) {
-  return (!p.dead && c.best != ~0ull && (int)(c.best >> 32) < p.hnew) ? c.res : 0;
-}
-
-// One discharge step of u by a tile of T lanes (T <= 64, inside one wave).
+// One discharge step of u by a tile of T lanes (T < kMultiPushLanes, inside one wave): the
+// lowest residual arc gets min(e(u), r) after a relabel to its height + 1 if it is not lower.
 // Returns the vertex the tile continues with (push forwarding, see finish_vertex) or -1; fcls is
 // the forwarding class limit (-1: none).
 template <int T, class L>
@@ -229,21 +277,12 @@ __device__ __forceinline__ int discharge_tile(const L &g, const CycleArgs &a, St
   Plan p;
   const unsigned long long best = tile_min<T>(c.best);
   plan_heights(best, hu, a.H, p);
-  if constexpr (T >= kMultiPushLanes) {
-    const long long want = admissible_residual(p, c);
-    const long long incl = tile_scan<T>(want);
-    const long long sum = __shfl(incl, T - 1, T);
-    p.d = (int)max(0LL, min(want, (long long)eu - (incl - want)));
-    p.total = (int)min((long long)eu, sum);
-    fcls = -1;  // several lanes push: no single vertex to continue with
-  } else {
-    // low-degree vertices push along the minimum arc only: fanning small excesses out to many
-    // neighbours (unit-capacity bipartite networks) multiplies the pushes that follow
-    const bool winner = !p.dead && c.best == best;  // one lane: keys are unique
-    const int total = (int)tile_sum<T>(winner ? (long long)min(eu, c.res) : 0LL);
-    p.d = winner ? total : 0;
-    p.total = total;
-  }
+  // low-degree vertices push along the minimum arc only: fanning small excesses out to many
+  // neighbours (unit-capacity bipartite networks) multiplies the pushes that follow
+  const bool winner = !p.dead && c.best == best;  // one lane: keys are unique
+  const int total = (int)tile_sum<T>(winner ? (long long)min(eu, c.res) : 0LL);
+  p.d = winner ? total : 0;
+  p.total = total;
   const int f = finish_vertex(g, a, S, u, hu, p, c, lane == 0, deg, fcls, st);
   if constexpr (T == 1) {
     return f;
@@ -252,33 +291,22 @@ __device__ __forceinline__ int discharge_tile(const L &g, const CycleArgs &a, St
   }
 }
 
-// One discharge step of a hub vertex by the whole block (kBlock lanes, LDS reductions).
-template <class L>
-__device__ __forceinline__ void discharge_block(const L &g, const CycleArgs &a, Stage *S, int u,
-                                                Stats &st) {
+// Block-wide minimum (kBlock threads, all get the result).
+__device__ __forceinline__ unsigned long long block_min(unsigned long long v) {
   __shared__ unsigned long long red[kBlock / 64];
+  v = tile_min<64>(v);
+  if (__lane_id() == 0) red[threadIdx.x / 64] = v;
+  __syncthreads();
+  for (int k = 0; k < kBlock / 64; ++k) v = red[k] < v ? red[k] : v;
+  __syncthreads();
+  return v;
+}
+
+// Block-wide inclusive prefix sum of v in thread order; total = the sum over the block.
+__device__ __forceinline__ long long block_scan(long long v, long long &total) {
   __shared__ long long wsum[kBlock / 64];
-  __shared__ int s_eu;
-  // e(u) grows while the block runs (other vertices push into u): one thread reads it and the
-  // block shares that value, so every wave splits the same excess as the one subtracted
-  if (threadIdx.x == 0) s_eu = a.e[u];
-  __syncthreads();
-  const int eu = s_eu;
-  const int hu = a.h[u];
-  __syncthreads();
-  if (hu >= a.H) return;  // uniform over the block: only this block writes h[u]
-  Cand c;
-  int deg;
-  scan_slots<scan_batch<kBlock>()>(g, a.h, u, (int)threadIdx.x, kBlock, c, deg);
   const int wave = threadIdx.x / 64;
-  unsigned long long best = tile_min<64>(c.best);
-  if (__lane_id() == 0) red[wave] = best;
-  __syncthreads();
-  for (int k = 0; k < kBlock / 64; ++k) best = red[k] < best ? red[k] : best;
-  Plan p;
-  plan_heights(best, hu, a.H, p);
-  const long long want = admissible_residual(p, c);
-  long long incl = tile_scan<64>(want);
+  long long incl = tile_scan<64>(v);
   if (__lane_id() == 63) wsum[wave] = incl;
   __syncthreads();
   long long before = 0, sum = 0;
@@ -287,10 +315,304 @@ __device__ __forceinline__ void discharge_block(const L &g, const CycleArgs &a, 
     sum += wsum[k];
   }
   __syncthreads();
-  incl += before;
-  p.d = (int)max(0LL, min(want, (long long)eu - (incl - want)));
-  p.total = (int)min((long long)eu, sum);
-  finish_vertex(g, a, S, u, hu, p, c, threadIdx.x == 0, deg, -1, st);
+  total = sum;
+  return incl + before;
+}
+
+// Reductions of the group of lanes that discharges one vertex together: a tile inside one wave
+// (shuffles) or the whole block (LDS).  Every lane of the group calls them.
+template <int T>
+struct Group {
+  __device__ static unsigned long long min(unsigned long long v) { return tile_min<T>(v); }
+  __device__ static long long scan(long long v, long long &total) {
+    const long long incl = tile_scan<T>(v);
+    total = __shfl(incl, T - 1, T);
+    return incl;
+  }
+  __device__ static int first(int v) { return __shfl(v, 0, T); }  // lane 0's value
+};
+template <>
+struct Group<kBlock> {
+  __device__ static unsigned long long min(unsigned long long v) { return block_min(v); }
+  __device__ static long long scan(long long v, long long &total) { return block_scan(v, total); }
+  __device__ static int first(int v) {
+    __shared__ int s_v;
+    if (threadIdx.x == 0) s_v = v;
+    __syncthreads();
+    v = s_v;
+    __syncthreads();
+    return v;
+  }
+};
+
+// This lane pushes d along the arc (key) to w, whose height was hw; true when w must be queued
+// (first excess, not a sink).
+template <class L>
+__device__ __forceinline__ bool push_to(const L &g, const CycleArgs &a, int w, int key, int hw,
+                                        int d, Stats &st) {
+  const int old_w = atomicAdd(&a.e[w], d);
+  g.push(key, d);
+  st.pushes++;
+  return old_w == 0 && hw > 0;
+}
+
+// One discharge step of a vertex by a group of T lanes that pushes along many arcs (a 64-lane
+// tile or the whole block), chunk by chunk.  A chunk is one batch of B slots per lane: the lane
+// loads its B slot records, then their B neighbour heights, and pushes its share of what is
+// left of e(u) (a prefix split over the lanes) along every one of its arcs that is lower than
+// h(u).  The scan stops as soon as the excess is placed, so the admissible arcs of the first
+// chunks usually take it all and the rest of a hub's slots -- a random neighbour-height gather
+// each -- are not read.  Only a full scan without any arc below h(u) relabels: u moves to the
+// lowest neighbour + 1, and every lane whose lowest arc of the whole scan sits at that height
+// pushes along it (relabel and push in one step).  Arcs that went lower than h(u) by a stale
+// read, or arcs beyond the lane's lowest, are admissible pushes of the same kind the lock-free
+// rule allows (the neighbour was lower when read).  Queueing follows finish_vertex: the owner
+// re-adds u iff excess remains, pushers add w iff they gave it its first excess.
+constexpr int kMultiBatch = 4;  // slots per lane per chunk (a block's chunk: 1,024 slots)
+
+template <int T, class L>
+__device__ __forceinline__ void discharge_multi(const L &g, const CycleArgs &a, Stage *S, int u,
+                                                int lane, Stats &st) {
+  constexpr int B = kMultiBatch;
+  const int hu = a.h[u];
+  // e(u) grows while the group runs: all lanes split lane 0's read (the value subtracted below)
+  const int eu = Group<T>::first(lane == 0 ? a.e[u] : 0);
+  if (hu >= a.H) return;  // became inactive after it was queued (uniform)
+  Cand keep;              // this lane's lowest arc over the whole scan
+  long long placed = 0;   // excess pushed so far (the same in every lane)
+  int scanned = 0;
+#pragma unroll
+  for (int seg = 0; seg < L::kSeg; ++seg) {
+    const int2 r = g.range(u, seg);
+    for (int cs = r.x; cs < r.y && placed < eu; cs += T * B) {
+      int w[B], res[B], key[B], hw[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        w[b] = u;
+        res[b] = 0;
+        key[b] = 0;
+        if (cs + lane + b * T < r.y) g.scan(seg, cs + lane + b * T, w[b], res[b], key[b]);
+      }
+#pragma unroll
+      for (int b = 0; b < B; ++b) hw[b] = (res[b] > 0 && w[b] != u) ? a.h[w[b]] : 0;
+      long long want = 0;
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        if (res[b] <= 0 || w[b] == u) continue;
+        keep.consider(hw[b], key[b], w[b], res[b]);
+        if (hw[b] < hu) want += res[b];
+      }
+      scanned += min(r.y - cs, T * B);
+      long long sum;
+      const long long incl = Group<T>::scan(want, sum);
+      const long long left = eu - placed;
+      long long d = max(0LL, min(want, left - (incl - want)));
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        bool add = false;
+        if (d > 0 && res[b] > 0 && w[b] != u && hw[b] < hu) {
+          const int x = (int)min(d, (long long)res[b]);
+          d -= x;
+          add = push_to(g, a, w[b], key[b], hw[b], x, st);
+        }
+        queue_next(a, S, w[b], add);
+      }
+      placed += min(left, sum);
+    }
+  }
+  Plan p;
+  p.dead = false;
+  p.hnew = hu;
+  if (placed == 0 && eu > 0) {  // a full scan found no arc below h(u): relabel
+    plan_heights(Group<T>::min(keep.best), hu, a.H, p);
+    if (!p.dead) {
+      const long long want =
+          (keep.best != ~0ull && (int)(keep.best >> 32) < p.hnew) ? keep.res : 0;
+      long long sum;
+      const long long incl = Group<T>::scan(want, sum);
+      const int d = (int)max(0LL, min(want, (long long)eu - (incl - want)));
+      queue_next(a, S, keep.w, push_arc(g, a, keep, d, st));
+      placed = min((long long)eu, sum);
+    }
+  }
+  bool want_u = false;
+  if (lane == 0) {
+    st.active++;
+    st.slots += scanned;
+    if (p.dead || p.hnew > hu) {
+      a.h[u] = p.hnew;
+      st.relabels++;
+    }
+    if (!p.dead) want_u = atomicSub(&a.e[u], (int)placed) - placed > 0;
+  }
+  queue_next(a, S, u, want_u);
+}
+
+// One discharge step of u by T lanes; returns the vertex to continue with (forwarding) or -1.
+template <int T, class L>
+__device__ __forceinline__ int discharge(const L &g, const CycleArgs &a, Stage *S, int u,
+                                         int lane, int fcls, Stats &st) {
+  if constexpr (T >= kMultiPushLanes) {
+    discharge_multi<T>(g, a, S, u, lane, st);
+    return -1;  // several lanes push: no single vertex to continue with
+  } else {
+    return discharge_tile<T>(g, a, S, u, lane, fcls, st);
+  }
+}
+
+// Takes up to want units from *e, keeping at least `keep` there; returns what it took and sets
+// left to the excess after the take.
+__device__ __forceinline__ int claim_excess(int *e, long long want, int keep, int &left) {
+  int old = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (;;) {
+    const int take = (int)max(0LL, min(want, (long long)old - keep));
+    if (take == 0) {
+      left = old;
+      return 0;
+    }
+    const int prev = atomicCAS(e, old, old - take);
+    if (prev == old) {
+      left = old - take;
+      return take;
+    }
+    old = prev;
+  }
+}
+
+// One discharge step of part p (of P) of giant u, by the whole block; i = u's position in the
+// cycle's giant list.  The parts of u run in any order, on any blocks, without waiting for each
+// other:
+//   - every part reads h(u) (only the last part writes it, after all parts have read it), scans
+//     its kPartSlots slots and pushes along its arcs to neighbours lower than h(u) -- those
+//     are admissible whatever the other parts find -- taking the excess by compare-and-swap
+//     but leaving at least one unit, so e(u) stays positive and u's worklist ticket stays with
+//     its owner;
+//   - it then folds its lowest (height, key) into gbest[i] and counts itself in garrive[i];
+//   - the part that arrives last sees every part's minimum: it relabels u if no neighbour is
+//     lower (or marks it dead), pushes what its own arcs still admit with a take that may
+//     empty e(u), then pushes what is left along u's lowest arc (whichever part holds it), so
+//     the step ends with e(u) empty or that arc saturated, as a single-block step does; it
+//     re-adds u iff excess remains -- the exactly-once rule of finish_vertex, applied once per
+//     discharge of u -- and resets gbest[i] and garrive[i] for the next cycle.
+template <class L>
+__device__ void discharge_part(const L &g, const CycleArgs &a, Stage *S, int u, int p, int P,
+                               int i, Stats &st) {
+  __shared__ int s_take, s_left, s_rank;
+  __shared__ unsigned long long s_best;
+  const int hu = a.h[u];
+  if (hu >= a.H) return;  // uniform over the parts: inactive since it was queued
+  const int deg = slot_degree(g, u);
+  const int lo = p * kPartSlots, hi = min(deg, lo + kPartSlots);
+  Cand c;
+  scan_range<scan_batch<kBlock>()>(g, a.h, u, lo, hi, (int)threadIdx.x, kBlock, c);
+  const unsigned long long best = block_min(c.best);
+  const long long want = (c.best != ~0ull && (int)(c.best >> 32) < hu) ? c.res : 0;
+  long long sum;
+  const long long incl = block_scan(want, sum);
+  if (threadIdx.x == 0) {
+    int left;
+    s_take = sum > 0 ? claim_excess(&a.e[u], sum, 1, left) : 0;
+  }
+  __syncthreads();
+  const int d = (int)max(0LL, min(want, (long long)s_take - (incl - want)));
+  queue_next(a, S, c.w, push_arc(g, a, c, d, st));
+  if (threadIdx.x == 0) {
+    st.slots += hi - lo;
+    atomicMin(&a.gbest[i], best);
+    __threadfence();  // the minimum and the pushes before the arrival
+    s_rank = atomicAdd(&a.garrive[i], 1);
+  }
+  __syncthreads();
+  if (s_rank != P - 1) return;  // uniform
+  __threadfence();
+  if (threadIdx.x == 0) {
+    s_best = atomicExch(&a.gbest[i], ~0ull);
+    atomicExch(&a.garrive[i], 0);
+  }
+  __syncthreads();
+  Plan pl;
+  plan_heights(s_best, hu, a.H, pl);
+  const long long more =
+      (!pl.dead && c.best != ~0ull && (int)(c.best >> 32) < pl.hnew) ? c.res - d : 0;
+  long long sum2;
+  const long long incl2 = block_scan(more, sum2);
+  if (threadIdx.x == 0) {
+    int left;
+    s_take = claim_excess(&a.e[u], sum2, 0, left);
+    s_left = left;
+    st.active++;
+    if (pl.dead || pl.hnew > hu) {
+      a.h[u] = pl.hnew;
+      st.relabels++;
+    }
+  }
+  __syncthreads();
+  const int d2 = (int)max(0LL, min(more, (long long)s_take - (incl2 - more)));
+  queue_next(a, S, c.w, push_arc(g, a, c, d2, st));
+  int w3 = u;
+  bool want_w3 = false;
+  if (threadIdx.x == 0 && !pl.dead && s_left > 0) {
+    // the lowest arc's residual as it is now: other parts may have pushed on it in this launch
+    // (an atomic load reads it from L2, past any older copy in this CU's cache)
+    const int key = (int)(unsigned)(s_best & 0xffffffffull);
+    const int r = __hip_atomic_load(g.res_ptr(key), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int left = s_left;
+    const int x = r > 0 ? claim_excess(&a.e[u], r, 0, left) : 0;
+    s_left = left;
+    if (x > 0) {
+      w3 = g.target(key);
+      const int old_w = atomicAdd(&a.e[w3], x);
+      g.push(key, x);
+      st.pushes++;
+      want_w3 = old_w == 0 && (int)(s_best >> 32) > 0;  // height 0 only at sinks
+    }
+  }
+  queue_next(a, S, w3, want_w3);
+  queue_next(a, S, u, threadIdx.x == 0 && !pl.dead && s_left > 0);
+  __syncthreads();  // the shared words are reused by the block's next part
+}
+
+// Giants in worklist class c: every giant has ceil(deg / kPartSlots) parts; part j of the list
+// (counted over the giants in list order) goes to block j mod nblk.  Giants are few (the
+// largest hubs), so each block prefix-sums the parts of up to kBlock * 4 of them in LDS.
+template <class L>
+__device__ __forceinline__ void cycle_giant(const L &g, const CycleArgs &a, Stage *S, int c,
+                                            int cnt, int b, int nblk, Stats &st) {
+  constexpr int kPer = 4, kChunk = kBlock * kPer;
+  __shared__ int s_pre[kChunk + 1];
+  const int base = a.in.cls[c];
+  for (int k0 = 0; k0 < cnt; k0 += kChunk) {
+    const int kc = min(kChunk, cnt - k0);
+    int parts[kPer], mine = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int x = (int)threadIdx.x * kPer + q;
+      parts[q] = x < kc ? (slot_degree(g, a.in.items[base + k0 + x]) + kPartSlots - 1) / kPartSlots
+                        : 0;
+      mine += parts[q];
+    }
+    long long total;
+    int pre = (int)(block_scan(mine, total) - mine);
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int x = (int)threadIdx.x * kPer + q;
+      if (x < kc) s_pre[x] = pre;
+      pre += parts[q];
+    }
+    if (threadIdx.x == 0) s_pre[kc] = (int)total;
+    __syncthreads();
+    for (int j = b; j < (int)total; j += nblk) {
+      int lo = 0, hi = kc;  // s_pre[lo] <= j < s_pre[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_pre[mid] <= j) lo = mid;
+        else hi = mid;
+      }
+      discharge_part(g, a, S, a.in.items[base + k0 + lo], j - s_pre[lo], s_pre[lo + 1] - s_pre[lo],
+                     k0 + lo, st);
+    }
+    __syncthreads();  // s_pre is rewritten for the next chunk
+  }
 }
 
 // The tile that takes a queued vertex stamps it with the cycle before its step, so no forwarder
@@ -317,20 +639,28 @@ __device__ __forceinline__ void cycle_class(const L &g, const CycleArgs &a, Stag
       int v = a.in.items[base + i];
       if (a.fwd_hops > 0 && !claim_queued<T>(a, v, lane)) continue;
       for (int hop = 0; v >= 0; ++hop)
-        v = discharge_tile<T>(g, a, S, v, lane, hop < a.fwd_hops ? c : -1, st);
+        v = discharge<T>(g, a, S, v, lane, hop < a.fwd_hops ? c : -1, st);
     }
   } else {
-    for (int i = b; i < cnt; i += nblk) discharge_block(g, a, S, a.in.items[base + i], st);
+    for (int i = b; i < cnt; i += nblk) discharge_multi<kBlock>(g, a, S, a.in.items[base + i], (int)threadIdx.x, st);
   }
 }
 
-// Block b of the grid serves degree class c when d.blk_begin[c] <= b < d.blk_begin[c+1].
-__device__ __forceinline__ int block_class(const Dispatch &d) {
+// The class, index within the class and class block count of this block.  Block rb serves
+// degree class c when d.blk_begin[c] <= rb < d.blk_begin[c+1], rb counted from the end of the
+// grid: blocks start in index order, so the classes of the longest blocks (hubs) start first and
+// the short ones fill in behind them instead of leaving the hubs as the launch's tail.
+struct BlockSlot {
+  int c, b, nblk;
+};
+__device__ __forceinline__ BlockSlot block_slot(const Dispatch &d) {
+  const int rb = (int)(gridDim.x - 1 - blockIdx.x);
   int c = 0;
 #pragma unroll
   for (int q = 1; q < kNumClasses; ++q)
-    if (q < d.nclass && (int)blockIdx.x >= d.blk_begin[q]) c = q;
-  return c;
+    if (q < d.nclass && rb >= d.blk_begin[q]) c = q;
+  const int b0 = pick(d.blk_begin, c);
+  return BlockSlot{c, rb - b0, pick(d.blk_begin, c + 1) - b0};
 }
 
 template <class L>
@@ -342,10 +672,8 @@ __global__ __launch_bounds__(kBlock) void k_cycle(L g, CycleArgs a, Dispatch d) 
     a.clear_cnt[threadIdx.x * kCtlStride] = 0;
   __shared__ Stage S;
   stage_init(S);
-  const int c = block_class(d);
-  const int b0 = pick(d.blk_begin, c);
-  const int nblk = pick(d.blk_begin, c + 1) - b0;
-  const int b = (int)blockIdx.x - b0;
+  const BlockSlot bs = block_slot(d);
+  const int c = bs.c, b = bs.b, nblk = bs.nblk;
   const int cnt = list_count(a.in, c);
   Stats st{0, 0, 0, 0};
   switch (pick(d.lanes, c)) {
@@ -356,6 +684,7 @@ __global__ __launch_bounds__(kBlock) void k_cycle(L g, CycleArgs a, Dispatch d) 
     case 16: cycle_class<16>(g, a, &S, c, cnt, b, nblk, st); break;
     case 32: cycle_class<32>(g, a, &S, c, cnt, b, nblk, st); break;
     case 64: cycle_class<64>(g, a, &S, c, cnt, b, nblk, st); break;
+    case kSplitLanes: cycle_giant(g, a, &S, c, cnt, b, nblk, st); break;
     default: cycle_class<kBlock>(g, a, &S, c, cnt, b, nblk, st); break;
   }
   if (a.work && __lane_id() == 0) {  // the wave's own work, before the block's flush
@@ -562,10 +891,8 @@ __global__ __launch_bounds__(kBlock) void k_bfs_level(L g, BfsArgs a, Dispatch d
   __shared__ Stage sf, sa;
   stage_init(sf);
   stage_init(sa);
-  const int c = block_class(d);
-  const int b0 = pick(d.blk_begin, c);
-  const int nblk = pick(d.blk_begin, c + 1) - b0;
-  const int b = (int)blockIdx.x - b0;
+  const BlockSlot bs = block_slot(d);
+  const int c = bs.c, b = bs.b, nblk = bs.nblk;
   // the two frontier lists share their class regions; only the item arrays alternate (selecting
   // whole descriptors would copy them to scratch memory)
   ListSet in = a.fr[0], out = a.fr[0];
@@ -667,11 +994,10 @@ __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(L g, BfsArgs a, Dispat
   stage_init(sf);
   stage_init(sa);
   const BfsStages S{&sf, &sa};
-  const int c = block_class(d);
-  const int b0 = pick(d.blk_begin, c);
-  const int nblk = pick(d.blk_begin, c + 1) - b0;
-  const int tid = ((int)blockIdx.x - b0) * kBlock + (int)threadIdx.x;
-  const int nthreads = nblk * kBlock;
+  const BlockSlot bs = block_slot(d);
+  const int c = bs.c;
+  const int tid = bs.b * kBlock + (int)threadIdx.x;
+  const int nthreads = bs.nblk * kBlock;
   ListSet out = a.fr[0];  // the frontier lists differ only in their item arrays
   out.items = (level & 1) ? a.fr[0].items : a.fr[1].items;
   out.cnt = pick(a.fcnt, (level + 1) % 3);
@@ -859,7 +1185,7 @@ __device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &c
       int v = ca.in.items[base + i];                                               \
       if (ca.fwd_hops > 0 && !claim_queued<TT>(ca, v, tid % TT)) continue;         \
       for (int hop = 0; v >= 0; ++hop)                                             \
-        v = discharge_tile<TT>(g, ca, nullptr, v, tid % TT, hop < ca.fwd_hops ? c : -1, st); \
+        v = discharge<TT>(g, ca, nullptr, v, tid % TT, hop < ca.fwd_hops ? c : -1, st); \
     }                                                                              \
     break;
     MF_BATCH_CASE(1)
@@ -871,7 +1197,7 @@ __device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &c
     MF_BATCH_CASE(64)
 #undef MF_BATCH_CASE
     default:
-      for (int i = 0; i < cnt; ++i) discharge_block(g, ca, nullptr, ca.in.items[base + i], st);
+      for (int i = 0; i < cnt; ++i) discharge_multi<kBlock>(g, ca, nullptr, ca.in.items[base + i], tid, st);
       break;
   }
 }
@@ -939,6 +1265,8 @@ __device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, i
   ca.cycle_base = nullptr;
   ca.work = nullptr;
   ca.work_cls = nullptr;
+  ca.gbest = nullptr;  // giants are discharged by the whole workgroup like other hubs
+  ca.garrive = nullptr;
   ca.cycle = 0;
 
   // preflow: saturate the source's residual arcs
@@ -1120,6 +1448,10 @@ static mf_status dmalloc(T **p, size_t count) {
   return MF_OK;
 }
 
+static_assert(kClasses == kNumClasses, "host and device class counts differ");
+static_assert(kTunedClasses == sizeof(((mf_options *)nullptr)->class_lanes) / sizeof(uint16_t),
+              "mf_options.class_lanes covers the tunable classes");
+
 ClassConfig class_config(mf_algo algo, int tile_lanes, const uint16_t *class_lanes) {
   ClassConfig c;
   if (algo == MF_ALGO_TC) {
@@ -1131,11 +1463,11 @@ ClassConfig class_config(mf_algo algo, int tile_lanes, const uint16_t *class_lan
     c.lanes[0] = tile_lanes;
     c.maxdeg[0] = INT32_MAX;
   } else {
-    c.nclass = 5;
-    const int lanes[5] = {8, 16, 32, 64, kBlock};
-    const int maxdeg[5] = {8, 16, 32, 1024, INT32_MAX};
-    for (int k = 0; k < 5; ++k) {
-      c.lanes[k] = (class_lanes && class_lanes[k]) ? class_lanes[k] : lanes[k];
+    c.nclass = kClasses;
+    const int lanes[kClasses] = {8, 16, 32, 64, kBlock, kSplitLanes};
+    const int64_t maxdeg[kClasses] = {8, 16, 32, 1024, kGiantSlots, INT64_MAX};
+    for (int k = 0; k < kClasses; ++k) {
+      c.lanes[k] = (k < kTunedClasses && class_lanes && class_lanes[k]) ? class_lanes[k] : lanes[k];
       c.maxdeg[k] = maxdeg[k];
     }
   }
@@ -1148,7 +1480,10 @@ int class_of_degree(const ClassConfig &c, int64_t deg) {
   return c.nclass - 1;
 }
 
-static Dispatch make_dispatch(const ClassConfig &c, const int64_t *pop, int num_cu) {
+// Blocks per class: enough tiles for the class population, at most 8 per CU; the split class
+// (giants) gets one block per part (split_units parts in all).
+static Dispatch make_dispatch(const ClassConfig &c, const int64_t *pop, int num_cu,
+                              int64_t split_units) {
   Dispatch d;
   memset(&d, 0, sizeof(d));
   d.nclass = c.nclass;
@@ -1161,6 +1496,8 @@ static Dispatch make_dispatch(const ClassConfig &c, const int64_t *pop, int num_
     if (c.lanes[k] <= 64) {
       const int64_t tpb = kBlock / c.lanes[k];
       nb = std::min<int64_t>((pop[k] + tpb - 1) / tpb, (int64_t)8 * num_cu);
+    } else if (c.lanes[k] == kSplitLanes) {
+      nb = std::min<int64_t>(std::max<int64_t>(split_units, pop[k]), (int64_t)8 * num_cu);
     } else {
       nb = std::min<int64_t>(pop[k], (int64_t)8 * num_cu);
     }
@@ -1252,6 +1589,9 @@ struct DevGraph {
   int64_t nonzero = 0;             // vertices of nonzero degree
   int64_t class_slots[kNumClasses] = {0};  // residual slots of the vertices of each VC class
   int64_t max_src_deg = 0;         // largest residual degree of a source
+  int64_t giant_parts = 0;         // parts of all giants (kPartSlots slots each)
+  unsigned long long *gbest = nullptr;  // per giant-list position: lowest (height, key) of the parts
+  int *garrive = nullptr;               //   and the parts done, both reset by the last part
   // host copies for result read-back
   std::vector<int32_t> h_off, h_dst, h_cap, h_fslot, h_src, h_snk;
   bool solved = false;
@@ -1281,7 +1621,7 @@ static void free_dev(DevGraph *g) {
   void *ptrs[] = {g->off, g->dc,  g->rev, g->dst,    g->cr,    g->fdc,   g->bcf,
                   g->roff, g->rdi, g->cap0, g->src,   g->snk,   g->values, g->h,
                   g->e,   g->lists, g->ctl, g->stats, g->vcls, g->vorder, g->stamp, g->fslot,
-                  g->work, g->work_cls, g->cycle_base, g->cls_tab};
+                  g->work, g->work_cls, g->cycle_base, g->cls_tab, g->gbest, g->garrive};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -1310,13 +1650,20 @@ static mf_status set_vertex_classes(DevGraph &G, const std::vector<int64_t> &deg
   std::vector<unsigned char> cls((size_t)n);
   for (int q = 0; q < kNumClasses; ++q) G.vc_pop[q] = G.bu_pop[q] = 0;
   for (int q = 0; q < kNumClasses; ++q) G.class_slots[q] = 0;
+  G.giant_parts = 0;
   for (int32_t v = 0; v < n; ++v) {
     const int q = class_of_degree(vc, degree[(size_t)v]);
     cls[(size_t)v] = (unsigned char)q;
     G.vc_pop[q]++;
     G.class_slots[q] += degree[(size_t)v];
     if (degree[(size_t)v] > 0) G.bu_pop[q]++;
+    if (vc.lanes[q] == kSplitLanes) G.giant_parts += (degree[(size_t)v] + kPartSlots - 1) / kPartSlots;
   }
+  // giant-list state, one entry per possible list position
+  const size_t giants = (size_t)std::max<int64_t>(G.vc_pop[kNumClasses - 1], 1);
+  if ((rc = dmalloc(&G.gbest, giants)) || (rc = dmalloc(&G.garrive, giants))) return rc;
+  HIP_TRY(hipMemsetAsync(G.gbest, 0xff, sizeof(unsigned long long) * giants, st));
+  HIP_TRY(hipMemsetAsync(G.garrive, 0, sizeof(int) * giants, st));
   G.nonzero = 0;
   int fill[kNumClasses];
   for (int q = 0; q < kNumClasses; ++q) {
@@ -1453,14 +1800,14 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   else
     for (int q = 0; q < kNumClasses; ++q) pop[q] = G.vc_pop[q];
   const unsigned char *vcls = cfg.nclass == 1 ? nullptr : G.vcls;
-  const Dispatch disp = make_dispatch(cfg, pop, ctx->num_cu);
+  const Dispatch disp = make_dispatch(cfg, pop, ctx->num_cu, G.giant_parts);
   const int grid = disp.blk_begin[cfg.nclass];
   // the global relabel always splits its frontier by degree class, whatever the cycle kernels do
   const ClassConfig bcfg = class_config(MF_ALGO_VC, 0);
-  const Dispatch bdisp = make_dispatch(bcfg, G.vc_pop, ctx->num_cu);
+  const Dispatch bdisp = make_dispatch(bcfg, G.vc_pop, ctx->num_cu, G.giant_parts);
   const int bgrid = bdisp.blk_begin[bcfg.nclass];
   // bottom-up levels walk the nonzero-degree vertices in class order
-  const Dispatch budisp = make_dispatch(bcfg, G.bu_pop, ctx->num_cu);
+  const Dispatch budisp = make_dispatch(bcfg, G.bu_pop, ctx->num_cu, 0);
   const int bugrid = budisp.blk_begin[bcfg.nclass];
   const bool bu_allowed = o.bfs_direction == 0 && G.nonzero >= kBottomUpMinVertices;
   const bool bu_always = o.bfs_direction == 2;
@@ -1526,6 +1873,8 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   ca.cycle_base = nullptr;
   ca.work = nullptr;
   ca.work_cls = nullptr;
+  ca.gbest = G.gbest;
+  ca.garrive = G.garrive;
   if (o.record_workload) {  // per-wave busy cycles, summed over this solve's cycle launches
     const int64_t waves = (int64_t)grid * (kBlock / 64);
     if (G.work_waves != waves) {
@@ -1859,7 +2208,7 @@ static mf_status check_options(const mf_options &o) {
     return v == 1 || v == 2 || v == 4 || v == 8 || v == 16 || v == 32 || v == 64 || v == kBlock;
   };
   if (o.tile_lanes != 0 && !lanes_ok(o.tile_lanes)) return MF_ERR_INVALID_ARGUMENT;
-  for (int k = 0; k < kNumClasses; ++k)
+  for (int k = 0; k < kTunedClasses; ++k)
     if (o.class_lanes[k] != 0 && !lanes_ok(o.class_lanes[k])) return MF_ERR_INVALID_ARGUMENT;
   if (o.cycles_per_relabel < 0 || o.max_rounds < 0 || o.bfs_block_frontier < 0)
     return MF_ERR_INVALID_ARGUMENT;
