@@ -59,10 +59,8 @@ struct CycleArgs {
   const int *cycle_base;  // graph replays: the chunk's first cycle number, set before each replay
   unsigned long long *work;  // optional: busy clock cycles of every wave, summed over launches
   unsigned char *work_cls;   // optional: degree class of every wave's block
-  // giants, per position in the cycle's giant list: the lowest (height, key) any part found and
-  // the parts that have finished (both back to ~0 / 0 once the last part is done)
-  unsigned long long *gbest;
-  int *garrive;
+  // giants: kGiantWords words of state per position in the cycle's giant list (discharge_part)
+  unsigned long long *giant;
 };
 
 // This cycle's number; a launch replayed from a captured graph adds the chunk's first cycle
@@ -381,7 +379,7 @@ __device__ __forceinline__ void discharge_multi(const L &g, const CycleArgs &a, 
   Cand keep;              // this lane's lowest arc over the whole scan
   long long placed = 0;   // excess pushed so far (the same in every lane)
   int scanned = 0;
-#pragma unroll
+#pragma unroll 1
   for (int seg = 0; seg < L::kSeg; ++seg) {
     const int2 r = g.range(u, seg);
     for (int cs = r.x; cs < r.y && placed < eu; cs += T * B) {
@@ -460,115 +458,151 @@ __device__ __forceinline__ int discharge(const L &g, const CycleArgs &a, Stage *
   }
 }
 
-// Takes up to want units from *e, keeping at least `keep` there; returns what it took and sets
-// left to the excess after the take.
-__device__ __forceinline__ int claim_excess(int *e, long long want, int keep, int &left) {
-  int old = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (;;) {
-    const int take = (int)max(0LL, min(want, (long long)old - keep));
-    if (take == 0) {
-      left = old;
-      return 0;
-    }
-    const int prev = atomicCAS(e, old, old - take);
-    if (prev == old) {
-      left = old - take;
-      return take;
-    }
-    old = prev;
-  }
+// Words of per-giant state, per position in the cycle's giant list: [0] the lowest (height, key)
+// any part found, [1] parts done (high 32 bits) and the excess they pushed (low 32 bits), [2]
+// admissible residual reserved so far.  The last part of a giant resets all three.
+constexpr int kGiantWords = 4;
+
+// x now depends on dep: the wave waits for dep (the result of an atomic) before anything that
+// uses x is issued.  An empty instruction; only the register dependency matters.
+__device__ __forceinline__ void order_after(unsigned long long &x, unsigned long long dep) {
+  asm volatile("" : "+v"(x) : "v"(dep));
 }
 
 // One discharge step of part p (of P) of giant u, by the whole block; i = u's position in the
 // cycle's giant list.  The parts of u run in any order, on any blocks, without waiting for each
-// other:
-//   - every part reads h(u) (only the last part writes it, after all parts have read it), scans
-//     its kPartSlots slots and pushes along its arcs to neighbours lower than h(u) -- those
-//     are admissible whatever the other parts find -- taking the excess by compare-and-swap
-//     but leaving at least one unit, so e(u) stays positive and u's worklist ticket stays with
-//     its owner;
-//   - it then folds its lowest (height, key) into gbest[i] and counts itself in garrive[i];
-//   - the part that arrives last sees every part's minimum: it relabels u if no neighbour is
-//     lower (or marks it dead), pushes what its own arcs still admit with a take that may
-//     empty e(u), then pushes what is left along u's lowest arc (whichever part holds it), so
-//     the step ends with e(u) empty or that arc saturated, as a single-block step does; it
-//     re-adds u iff excess remains -- the exactly-once rule of finish_vertex, applied once per
-//     discharge of u -- and resets gbest[i] and garrive[i] for the next cycle.
+// other and without fences:
+//   - every part reads h(u) (only the last part writes it, after all parts have read it) and
+//     e(u).  Only the last part subtracts from e(u), so every part's read is a lower bound of
+//     the excess it subtracts from.  The part scans its slots chunk by chunk; for the arcs of a
+//     chunk lower than h(u) it reserves their residual on word [2] (one atomicAdd) and pushes
+//     min(want, e_read - reserved before it).  Whichever part reserves last with a positive
+//     share stays within its own read, so all parts together push at most e(u), and the
+//     excess present when the parts started is placed if the arcs admit it.  A part stops
+//     scanning when its share falls short of its chunk's want (the excess is placed);
+//   - it then folds its lowest (height, key) into word [0] and reports itself and its pushed
+//     total on word [1].  Both are returning atomics, performed where device-wide atomics are
+//     ordered before they return; the report is issued only after the minimum returned;
+//   - the part whose report comes last sees every part's minimum and total: it relabels u if
+//     no part found a lower neighbour (then none pushed) and pushes along its own lowest arcs at
+//     the new height, subtracts everything pushed from e(u) in one atomic and re-adds u iff
+//     excess remains -- the exactly-once rule of finish_vertex: until that subtraction e(u)
+//     has not dropped, so no pusher has queued u.
 template <class L>
 __device__ void discharge_part(const L &g, const CycleArgs &a, Stage *S, int u, int p, int P,
                                int i, Stats &st) {
-  __shared__ int s_take, s_left, s_rank;
-  __shared__ unsigned long long s_best;
+  constexpr int B = kMultiBatch;
+  __shared__ long long s_share;
+  __shared__ unsigned long long s_word;
+  unsigned long long *gs = a.giant + (size_t)i * kGiantWords;
   const int hu = a.h[u];
   if (hu >= a.H) return;  // uniform over the parts: inactive since it was queued
+  const int lane = (int)threadIdx.x;
+  const int ep = Group<kBlock>::first(lane == 0 ? ld_acq(&a.e[u]) : 0);
   const int deg = slot_degree(g, u);
   const int lo = p * kPartSlots, hi = min(deg, lo + kPartSlots);
-  Cand c;
-  scan_range<scan_batch<kBlock>()>(g, a.h, u, lo, hi, (int)threadIdx.x, kBlock, c);
-  const unsigned long long best = block_min(c.best);
-  const long long want = (c.best != ~0ull && (int)(c.best >> 32) < hu) ? c.res : 0;
-  long long sum;
-  const long long incl = block_scan(want, sum);
-  if (threadIdx.x == 0) {
-    int left;
-    s_take = sum > 0 ? claim_excess(&a.e[u], sum, 1, left) : 0;
+  Cand keep;               // this lane's lowest arc over the part's scanned slots
+  long long pushed = 0;    // the part's pushes (the same in every lane)
+  bool more = true;
+  int scanned = 0, acc = 0;
+#pragma unroll 1
+  for (int seg = 0; seg < L::kSeg; ++seg) {
+    const int2 r = g.range(u, seg);
+    const int len = r.y - r.x;
+    const int s0 = r.x + max(lo - acc, 0), s1 = r.x + min(hi - acc, len);
+    acc += len;
+    for (int cs = s0; cs < s1 && more; cs += kBlock * B) {
+      int w[B], res[B], key[B], hw[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        w[b] = u;
+        res[b] = 0;
+        key[b] = 0;
+        if (cs + lane + b * kBlock < s1) g.scan(seg, cs + lane + b * kBlock, w[b], res[b], key[b]);
+      }
+#pragma unroll
+      for (int b = 0; b < B; ++b) hw[b] = (res[b] > 0 && w[b] != u) ? a.h[w[b]] : 0;
+      long long want = 0;
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        if (res[b] <= 0 || w[b] == u) continue;
+        keep.consider(hw[b], key[b], w[b], res[b]);
+        if (hw[b] < hu) want += res[b];
+      }
+      scanned += min(s1 - cs, kBlock * B);
+      long long sum;
+      const long long incl = block_scan(want, sum);
+      if (lane == 0) {
+        long long share = 0;
+        if (sum > 0) {
+          const long long got = (long long)atomicAdd(&gs[2], (unsigned long long)sum);
+          share = max(0LL, min(sum, (long long)ep - got));
+        }
+        s_share = share;
+      }
+      __syncthreads();
+      const long long share = s_share;
+      __syncthreads();
+      long long d = max(0LL, min(want, share - (incl - want)));
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        bool add = false;
+        if (d > 0 && res[b] > 0 && w[b] != u && hw[b] < hu) {
+          const int x = (int)min(d, (long long)res[b]);
+          d -= x;
+          add = push_to(g, a, w[b], key[b], hw[b], x, st);
+        }
+        queue_next(a, S, w[b], add);
+      }
+      pushed += share;
+      more = share == sum;
+    }
+  }
+  const unsigned long long best = block_min(keep.best);
+  if (lane == 0) {
+    st.slots += scanned;
+    const unsigned long long prev = atomicMin(&gs[0], best);
+    unsigned long long report = (1ull << 32) | (unsigned long long)pushed;
+    order_after(report, prev);
+    s_word = atomicAdd(&gs[1], report);
   }
   __syncthreads();
-  const int d = (int)max(0LL, min(want, (long long)s_take - (incl - want)));
-  queue_next(a, S, c.w, push_arc(g, a, c, d, st));
-  if (threadIdx.x == 0) {
-    st.slots += hi - lo;
-    atomicMin(&a.gbest[i], best);
-    __threadfence();  // the minimum and the pushes before the arrival
-    s_rank = atomicAdd(&a.garrive[i], 1);
-  }
+  const unsigned long long word = s_word;
   __syncthreads();
-  if (s_rank != P - 1) return;  // uniform
-  __threadfence();
-  if (threadIdx.x == 0) {
-    s_best = atomicExch(&a.gbest[i], ~0ull);
-    atomicExch(&a.garrive[i], 0);
+  if ((int)(word >> 32) != P - 1) return;  // uniform
+  // the last part: every other part has reported
+  const long long total = (long long)(word & 0xffffffffull) + pushed;
+  if (lane == 0) {
+    unsigned long long b0 = atomicExch(&gs[0], ~0ull);
+    s_word = b0;
+    atomicExch(&gs[1], 0ull);
+    atomicExch(&gs[2], 0ull);
   }
   __syncthreads();
   Plan pl;
-  plan_heights(s_best, hu, a.H, pl);
-  const long long more =
-      (!pl.dead && c.best != ~0ull && (int)(c.best >> 32) < pl.hnew) ? c.res - d : 0;
-  long long sum2;
-  const long long incl2 = block_scan(more, sum2);
-  if (threadIdx.x == 0) {
-    int left;
-    s_take = claim_excess(&a.e[u], sum2, 0, left);
-    s_left = left;
+  plan_heights(s_word, hu, a.H, pl);
+  long long extra = 0;
+  if (!pl.dead && pl.hnew > hu) {
+    // relabel (no part found a lower neighbour, so none pushed): this part's lowest arcs at the
+    // new height are admissible now
+    const int en = Group<kBlock>::first(lane == 0 ? ld_acq(&a.e[u]) : 0);
+    const long long want = (keep.best != ~0ull && (int)(keep.best >> 32) < pl.hnew) ? keep.res : 0;
+    long long sum;
+    const long long incl = block_scan(want, sum);
+    const int d = (int)max(0LL, min(want, (long long)en - (incl - want)));
+    queue_next(a, S, keep.w, push_arc(g, a, keep, d, st));
+    extra = min((long long)en, sum);
+  }
+  bool want_u = false;
+  if (lane == 0) {
     st.active++;
     if (pl.dead || pl.hnew > hu) {
       a.h[u] = pl.hnew;
       st.relabels++;
     }
+    if (!pl.dead) want_u = atomicSub(&a.e[u], (int)(total + extra)) - (total + extra) > 0;
   }
-  __syncthreads();
-  const int d2 = (int)max(0LL, min(more, (long long)s_take - (incl2 - more)));
-  queue_next(a, S, c.w, push_arc(g, a, c, d2, st));
-  int w3 = u;
-  bool want_w3 = false;
-  if (threadIdx.x == 0 && !pl.dead && s_left > 0) {
-    // the lowest arc's residual as it is now: other parts may have pushed on it in this launch
-    // (an atomic load reads it from L2, past any older copy in this CU's cache)
-    const int key = (int)(unsigned)(s_best & 0xffffffffull);
-    const int r = __hip_atomic_load(g.res_ptr(key), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    int left = s_left;
-    const int x = r > 0 ? claim_excess(&a.e[u], r, 0, left) : 0;
-    s_left = left;
-    if (x > 0) {
-      w3 = g.target(key);
-      const int old_w = atomicAdd(&a.e[w3], x);
-      g.push(key, x);
-      st.pushes++;
-      want_w3 = old_w == 0 && (int)(s_best >> 32) > 0;  // height 0 only at sinks
-    }
-  }
-  queue_next(a, S, w3, want_w3);
-  queue_next(a, S, u, threadIdx.x == 0 && !pl.dead && s_left > 0);
+  queue_next(a, S, u, want_u);
   __syncthreads();  // the shared words are reused by the block's next part
 }
 
@@ -1265,8 +1299,7 @@ __device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, i
   ca.cycle_base = nullptr;
   ca.work = nullptr;
   ca.work_cls = nullptr;
-  ca.gbest = nullptr;  // giants are discharged by the whole workgroup like other hubs
-  ca.garrive = nullptr;
+  ca.giant = nullptr;  // giants are discharged by the whole workgroup like other hubs
   ca.cycle = 0;
 
   // preflow: saturate the source's residual arcs
@@ -1590,8 +1623,7 @@ struct DevGraph {
   int64_t class_slots[kNumClasses] = {0};  // residual slots of the vertices of each VC class
   int64_t max_src_deg = 0;         // largest residual degree of a source
   int64_t giant_parts = 0;         // parts of all giants (kPartSlots slots each)
-  unsigned long long *gbest = nullptr;  // per giant-list position: lowest (height, key) of the parts
-  int *garrive = nullptr;               //   and the parts done, both reset by the last part
+  unsigned long long *giant = nullptr;  // per giant-list position: state of the parts
   // host copies for result read-back
   std::vector<int32_t> h_off, h_dst, h_cap, h_fslot, h_src, h_snk;
   bool solved = false;
@@ -1621,7 +1653,7 @@ static void free_dev(DevGraph *g) {
   void *ptrs[] = {g->off, g->dc,  g->rev, g->dst,    g->cr,    g->fdc,   g->bcf,
                   g->roff, g->rdi, g->cap0, g->src,   g->snk,   g->values, g->h,
                   g->e,   g->lists, g->ctl, g->stats, g->vcls, g->vorder, g->stamp, g->fslot,
-                  g->work, g->work_cls, g->cycle_base, g->cls_tab, g->gbest, g->garrive};
+                  g->work, g->work_cls, g->cycle_base, g->cls_tab, g->giant};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -1659,11 +1691,14 @@ static mf_status set_vertex_classes(DevGraph &G, const std::vector<int64_t> &deg
     if (degree[(size_t)v] > 0) G.bu_pop[q]++;
     if (vc.lanes[q] == kSplitLanes) G.giant_parts += (degree[(size_t)v] + kPartSlots - 1) / kPartSlots;
   }
-  // giant-list state, one entry per possible list position
+  // giant-list state, one entry per possible list position: no minimum, no parts, nothing
+  // reserved (each giant's last part restores this)
   const size_t giants = (size_t)std::max<int64_t>(G.vc_pop[kNumClasses - 1], 1);
-  if ((rc = dmalloc(&G.gbest, giants)) || (rc = dmalloc(&G.garrive, giants))) return rc;
-  HIP_TRY(hipMemsetAsync(G.gbest, 0xff, sizeof(unsigned long long) * giants, st));
-  HIP_TRY(hipMemsetAsync(G.garrive, 0, sizeof(int) * giants, st));
+  if ((rc = dmalloc(&G.giant, giants * kGiantWords))) return rc;
+  std::vector<unsigned long long> gs0(giants * kGiantWords, 0ull);
+  for (size_t q = 0; q < giants; ++q) gs0[q * kGiantWords] = ~0ull;
+  HIP_TRY(hipMemcpy(G.giant, gs0.data(), sizeof(unsigned long long) * gs0.size(),
+                    hipMemcpyHostToDevice));
   G.nonzero = 0;
   int fill[kNumClasses];
   for (int q = 0; q < kNumClasses; ++q) {
@@ -1873,8 +1908,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   ca.cycle_base = nullptr;
   ca.work = nullptr;
   ca.work_cls = nullptr;
-  ca.gbest = G.gbest;
-  ca.garrive = G.garrive;
+  ca.giant = G.giant;
   if (o.record_workload) {  // per-wave busy cycles, summed over this solve's cycle launches
     const int64_t waves = (int64_t)grid * (kBlock / 64);
     if (G.work_waves != waves) {
