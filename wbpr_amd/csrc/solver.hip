@@ -1893,6 +1893,10 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   ba.state = G.ctl + CTL_STATE;
   for (int c = 0; c < kNumClasses; ++c)
     ba.cls_mean[c] = G.vc_pop[c] ? (int)((G.class_slots[c] + G.vc_pop[c] - 1) / G.vc_pop[c]) : 0;
+  // a giant in the frontier always goes to the grid-wide levels: one workgroup would scan its
+  // (up to millions of) slots alone
+  for (int c = 0; c < bcfg.nclass; ++c)
+    if (bcfg.lanes[c] == kSplitLanes) ba.cls_mean[c] = std::max(ba.cls_mean[c], kWgBfsSlots + 1);
   ba.wg_slots = kWgBfsSlots;
   ba.stats = G.stats;
 
