@@ -1056,62 +1056,272 @@ __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(L g, BfsArgs a, Dispat
   block_flush(nv, ns, 0, 0, a.stats, ST_BU_VERTS);
 }
 
-// All BFS levels inside one workgroup while the frontier holds at most fmax vertices: workgroup
-// barriers replace kernel launches, which is what long-diameter graphs need.  The frontier
-// counters live in LDS for the duration (the frontier items stay in global memory, which the
-// workgroup's waves share through their common CU); on exit the current counters are handed
-// back to the grid-wide kernels.  No device-scope fence is needed: every value this kernel
-// reads that another kernel wrote was written before the launch.
+// ---- one-workgroup BFS ----
+//
+// All BFS levels inside one workgroup while the frontier is small: workgroup barriers replace
+// kernel launches, which is what long-diameter graphs (thousands of levels) need, and a level
+// costs only its dependent memory round trips.  The frontier lives in LDS in three groups by
+// degree class -- up to 16 slots (tiles of 4 lanes), up to 1,024 (one wave per vertex), more
+// (the whole workgroup per vertex) -- and, for one-segment layouts, every entry carries its slot
+// range, loaded by the lane that claims the vertex together with its claim.  A level is then
+// three round trips (slot records; reverse residuals with the neighbours' heights, excesses and
+// classes; claims with the claimed vertices' ranges) instead of five (frontier item and offsets
+// first).  When the next frontier outgrows a group or the hand-over limits, its vertices go to
+// the device frontier lists by class, as the grid-wide kernels read them, and the kernel
+// returns.  No device-scope fence is needed: every value this kernel reads that another kernel
+// wrote was written before the launch.
+
+constexpr int kWgCapA = 2048, kWgCapB = 256, kWgCapC = 32;  // LDS frontier entries per group
+constexpr int kWgCap = kWgCapA + kWgCapB + kWgCapC;
+
+__device__ __forceinline__ int wg_group(int cls) { return cls <= 1 ? 0 : (cls <= 3 ? 1 : 2); }
+__device__ __forceinline__ int wg_cap(int q) { return q == 0 ? kWgCapA : (q == 1 ? kWgCapB : kWgCapC); }
+__device__ __forceinline__ int wg_off(int q) { return q == 0 ? 0 : (q == 1 ? kWgCapA : kWgCapA + kWgCapB); }
+
+// Where a level's claimed vertices go: the LDS groups of the next buffer, or (when a group is
+// full) the device list of the next level, counted in LDS.
+struct WgOut {
+  int *u, *b, *e;  // next buffer's entries (b, e: slot ranges, one-segment layouts only)
+  int *n;          // entries per group (may count past the capacity: readers clamp)
+  ListSet dev;     // device list of the next level (items in global memory, counters in LDS)
+  int *ovf;        // set when any vertex went to the device list
+  int nclass;
+};
+
+// Wave-aggregated append of the claimed w (class cw, slot range [rb, re)) to its LDS group;
+// past the group's capacity it goes to the device list.  Every lane of the wave calls this.
+template <bool kPre>
+__device__ __forceinline__ void wg_append(const WgOut &o, int w, bool want, int cw, int rb,
+                                          int re) {
+  if (__ballot(want) == 0) return;
+  const int lane = __lane_id();
+  const int grp = wg_group(cw);
+  bool spill = false;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const bool mine = want && grp == q;
+    const unsigned long long mask = __ballot(mine);
+    if (mask == 0) continue;
+    const int leader = __ffsll((long long)mask) - 1;
+    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&o.n[q], __popcll(mask));
+    base = __shfl(base, leader);
+    if (mine) {
+      const int pos = base + rank;
+      if (pos < wg_cap(q)) {
+        o.u[wg_off(q) + pos] = w;
+        if constexpr (kPre) {
+          o.b[wg_off(q) + pos] = rb;
+          o.e[wg_off(q) + pos] = re;
+        }
+      } else {
+        spill = true;
+      }
+    }
+  }
+  append(o.dev, w, spill, cw, o.nclass);
+  if (spill) *o.ovf = 1;
+}
+
+// T lanes (lane = 0..T-1) expand frontier vertex u, whose slot range is [rb, re) for one-segment
+// layouts; est collects the claimed vertices' estimated slots.
+template <int T, class L>
+__device__ __forceinline__ void wg_expand(const L &g, const BfsArgs &a, const WgOut &o, int u,
+                                          int rb, int re, int lane, int level, long long &ns,
+                                          long long &est) {
+  constexpr bool kPre = L::kSeg == 1;
+  constexpr int B = kBfsBatch;
+#pragma unroll 1
+  for (int seg = 0; seg < L::kSeg; ++seg) {
+    const int2 r = kPre ? make_int2(rb, re) : g.range(u, seg);
+    if (lane == 0) ns += r.y - r.x;
+    for (int base = r.x + lane; base < r.y; base += T * B) {
+      int w[B], idx[B], rin[B], hw[B], ew[B], cw[B], aw[B], wb[B], we[B];
+      bool claim[B];
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        w[k] = -1;
+        idx[k] = 0;
+        if (base + k * T < r.y) g.bfs_slot(seg, base + k * T, w[k], idx[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        const bool ok = w[k] >= 0 && w[k] != u;
+        rin[k] = ok ? g.bfs_rin(seg, idx[k]) : 0;
+        hw[k] = ok ? a.h[w[k]] : -1;
+        ew[k] = ok ? a.e[w[k]] : 0;
+        cw[k] = (ok && a.vcls) ? a.vcls[w[k]] : 0;
+        aw[k] = (ok && a.acls) ? a.acls[w[k]] : 0;
+      }
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        const bool cand = rin[k] > 0 && hw[k] == a.H;
+        wb[k] = 0;
+        we[k] = 0;
+        if constexpr (kPre) {
+          if (cand) {  // the range travels with the entry: loaded beside the claim
+            const int2 rr = g.range(w[k], 0);
+            wb[k] = rr.x;
+            we[k] = rr.y;
+          }
+        }
+        if (rin[k] > 0 && hw[k] == a.H + 1) a.flags[0] = 1;  // residual path from a source
+        claim[k] = cand && atomicCAS(&a.h[w[k]], a.H, level + 1) == a.H;
+        if (claim[k]) est += pick(a.cls_mean, cw[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        wg_append<kPre>(o, w[k], claim[k], cw[k], wb[k], we[k]);
+        append(a.active, w[k], claim[k] && ew[k] > 0, aw[k], a.nclass_act);
+      }
+    }
+  }
+}
+
 template <class L>
 __global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d, int level,
                                                      int fmax) {
-  __shared__ int s_cnt[3][kNumClasses];
-  __shared__ int s_size, s_big;
-  long long nv = 0, ns = 0;
+  constexpr bool kPre = L::kSeg == 1;
+  __shared__ int s_u[2][kWgCap];
+  __shared__ int s_b[kPre ? 2 : 1][kPre ? kWgCap : 1];
+  __shared__ int s_e[kPre ? 2 : 1][kPre ? kWgCap : 1];
+  __shared__ int s_n[2][3];
+  __shared__ int s_ocnt[kNumClasses];
+  __shared__ int s_ovf, s_go, s_size;
+  __shared__ long long s_est;
   const int tid = (int)threadIdx.x;
-  if (tid < 3 * kNumClasses) {
-    const int q = tid / kNumClasses, c = tid % kNumClasses;
-    s_cnt[q][c] = (q == level % 3) ? pick(a.fcnt, q)[c * kCtlStride] : 0;
-  }
-  __syncthreads();
-  while (true) {
-    if (tid == 0) {
-      int sz = 0;
-      long long slots = 0;  // estimated from the class means: hubs make a few vertices heavy
-      for (int c = 0; c < d.nclass; ++c) {
-        sz += s_cnt[level % 3][c];
-        slots += (long long)s_cnt[level % 3][c] * pick(a.cls_mean, c);
-      }
-      s_size = sz;
-      s_big = slots > a.wg_slots;
-    }
-    if (tid < kNumClasses) s_cnt[(level + 2) % 3][tid] = 0;  // next level's output counters
-    __syncthreads();
-    const int size = s_size;
-    if (size == 0 || size > fmax || s_big) break;
-    ListSet in = a.fr[0], out = a.fr[0];  // same class regions, alternating item arrays
-    in.items = (level & 1) ? a.fr[1].items : a.fr[0].items;
-    out.items = (level & 1) ? a.fr[0].items : a.fr[1].items;
-    in.cnt = s_cnt[level % 3];
-    in.cstride = 1;
-    out.cnt = s_cnt[(level + 1) % 3];
-    out.cstride = 1;
-    for (int c = 0; c < d.nclass; ++c) {
-      const int cnt = list_count(in, c);
-      if (cnt == 0) continue;
-      bfs_dispatch<true>(g, a, in, out, BfsStages{nullptr, nullptr}, c, pick(d.lanes, c), cnt, tid,
-                   kWgBlock, level, nv, ns);
-    }
-    __syncthreads();
-    ++level;
-  }
-  if (tid < 3 * kNumClasses) {
-    const int q = tid / kNumClasses, c = tid % kNumClasses;
-    pick(a.fcnt, q)[c * kCtlStride] = (q == level % 3) ? s_cnt[q][c] : 0;
+  long long nv = 0, ns = 0;
+  fmax = min(fmax, kWgCap);
+  // take over the current frontier from the device lists if it fits
+  ListSet in = a.fr[0];
+  in.items = (level & 1) ? a.fr[1].items : a.fr[0].items;
+  in.cnt = pick(a.fcnt, level % 3);
+  if (tid < 3) {
+    s_n[0][tid] = 0;
+    s_n[1][tid] = 0;
   }
   if (tid == 0) {
+    int n[3] = {0, 0, 0}, sz = 0;
+    long long slots = 0;  // estimated from the class means: hubs make a few vertices heavy
+    for (int c = 0; c < d.nclass; ++c) {
+      const int k = list_count(in, c);
+      sz += k;
+      slots += (long long)k * pick(a.cls_mean, c);
+      const int q = wg_group(c);
+      n[q] += k;
+    }
+    s_size = sz;
+    s_go = sz > 0 && sz <= fmax && slots <= a.wg_slots && n[0] <= kWgCapA && n[1] <= kWgCapB &&
+           n[2] <= kWgCapC;
+  }
+  __syncthreads();
+  if (!s_go) {  // leave the device lists as they are
+    if (tid == 0) {
+      a.state[0] = level;
+      a.state[1] = s_size;
+    }
+    return;
+  }
+  for (int c = 0; c < d.nclass; ++c) {
+    const int k = list_count(in, c), q = wg_group(c), off = in.cls[c];
+    for (int i = tid; i < k; i += kWgBlock) {
+      const int u = in.items[off + i];
+      const int pos = wg_off(q) + atomicAdd(&s_n[0][q], 1);  // fits: checked above
+      s_u[0][pos] = u;
+      if constexpr (kPre) {
+        const int2 r = g.range(u, 0);
+        s_b[0][pos] = r.x;
+        s_e[0][pos] = r.y;
+      }
+    }
+  }
+  int p = 0;
+  for (;;) {
+    // level `level` sits in buffer p; its claims go to buffer p ^ 1 (or the device list)
+    if (tid < 3) s_n[p ^ 1][tid] = 0;
+    if (tid < kNumClasses) s_ocnt[tid] = 0;
+    if (tid == 0) {
+      s_ovf = 0;
+      s_est = 0;
+    }
+    __syncthreads();
+    const int nA = min(s_n[p][0], kWgCapA), nB = min(s_n[p][1], kWgCapB),
+              nC = min(s_n[p][2], kWgCapC);
+    WgOut o;
+    o.u = s_u[p ^ 1];
+    o.b = kPre ? s_b[p ^ 1] : nullptr;
+    o.e = kPre ? s_e[p ^ 1] : nullptr;
+    o.n = s_n[p ^ 1];
+    o.dev = a.fr[0];
+    o.dev.items = (level & 1) ? a.fr[0].items : a.fr[1].items;  // the next level's item array
+    o.dev.cnt = s_ocnt;
+    o.dev.cstride = 1;
+    o.ovf = &s_ovf;
+    o.nclass = a.nclass;
+    long long est = 0;
+    const int* const cu = s_u[p];
+    const int* const cb = kPre ? s_b[p] : nullptr;
+    const int* const ce = kPre ? s_e[p] : nullptr;
+    for (int i = tid / 4; i < nA; i += kWgBlock / 4) {
+      const int x = wg_off(0) + i;
+      wg_expand<4>(g, a, o, cu[x], kPre ? cb[x] : 0, kPre ? ce[x] : 0, tid % 4, level, ns, est);
+      if (tid % 4 == 0) ++nv;
+    }
+    for (int i = tid / 64; i < nB; i += kWgBlock / 64) {
+      const int x = wg_off(1) + i;
+      wg_expand<64>(g, a, o, cu[x], kPre ? cb[x] : 0, kPre ? ce[x] : 0, tid % 64, level, ns, est);
+      if (tid % 64 == 0) ++nv;
+    }
+    for (int i = 0; i < nC; ++i) {
+      const int x = wg_off(2) + i;
+      wg_expand<kWgBlock>(g, a, o, cu[x], kPre ? cb[x] : 0, kPre ? ce[x] : 0, tid, level, ns, est);
+      if (tid == 0) ++nv;
+    }
+    est = tile_sum<64>(est);
+    if (__lane_id() == 0 && est) atomicAdd((unsigned long long *)&s_est, (unsigned long long)est);
+    __syncthreads();
+    ++level;
+    p ^= 1;
+    // the next frontier (now `level`) stays in LDS only if nothing spilled and it is small
+    if (tid == 0) {
+      const int sz = s_n[p][0] + s_n[p][1] + s_n[p][2];
+      s_size = sz;
+      s_go = !s_ovf && sz > 0 && sz <= fmax && s_est <= a.wg_slots;
+    }
+    __syncthreads();
+    if (s_go) continue;
+    break;
+  }
+  // hand the frontier of `level` to the device lists: what spilled is there already, the LDS
+  // entries follow (classes read back from the class array)
+  {
+    ListSet dev = a.fr[0];
+    dev.items = (level & 1) ? a.fr[1].items : a.fr[0].items;
+    dev.cnt = s_ocnt;
+    dev.cstride = 1;
+    for (int q = 0; q < 3; ++q) {
+      const int k = min(s_n[p][q], wg_cap(q));
+      for (int i0 = 0; i0 < k; i0 += kWgBlock) {
+        const int i = i0 + tid;
+        const bool want = i < k;
+        const int u = want ? s_u[p][wg_off(q) + i] : 0;
+        append(dev, u, want, (want && a.vcls) ? a.vcls[u] : 0, a.nclass);
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < kNumClasses) {
+    const int c = tid;
+    pick(a.fcnt, level % 3)[c * kCtlStride] = s_ocnt[c];
+    pick(a.fcnt, (level + 1) % 3)[c * kCtlStride] = 0;
+    pick(a.fcnt, (level + 2) % 3)[c * kCtlStride] = 0;
+  }
+  if (tid == 0) {
+    int sz = 0;
+    for (int c = 0; c < kNumClasses; ++c) sz += s_ocnt[c];
     a.state[0] = level;
-    a.state[1] = s_size;
+    a.state[1] = sz;
   }
   block_flush(nv, ns, 0, 0, a.stats, ST_BFS_VERTS);
 }
