@@ -49,14 +49,16 @@ PROFILE_EVERY = 16  # cycle launches per launch timed with dispatch-packet event
 WORKLOADS = {
     "genrmf64": dict(
         desc="Genrmf a=64 b=64 c1=1 c2=10000 seed=1 (262,144 vertices, 1,290,240 arcs)",
-        gen=("gen_genrmf", (64, 64, 1, 10000, 1)), value=19996127, algo="tc"),
+        gen=("gen_genrmf", (64, 64, 1, 10000, 1)), value=19996127, algo="vc",
+        variants=("tc", "vc")),
     "rmat22": dict(
         desc="R-MAT scale 22 edge factor 16 a=.57 b=.19 c=.19 caps [1,1000] seed=1 "
              "(4,194,304 vertices, 65,244,556 arcs after dropping self-loops and duplicates)",
         gen=("gen_rmat", (22, 16, 0.57, 0.19, 0.19, 1000, 1)), value=18886759, algo="vc"),
     "genrmf128": dict(
         desc="Genrmf a=128 b=128 c1=1 c2=10000 seed=1 (2,097,152 vertices, 10,403,840 arcs)",
-        gen=("gen_genrmf", (128, 128, 1, 10000, 1)), value=81048332, algo="tc"),
+        gen=("gen_genrmf", (128, 128, 1, 10000, 1)), value=81048332, algo="vc",
+        variants=("tc", "vc")),
     "rlg64": dict(
         desc="Washington RLG 64x64 r=10000 seed=1 (4,098 vertices, 12,224 arcs)",
         gen=("gen_washington_rlg", (64, 64, 10000, 1)), value=466088, algo="tc"),
@@ -202,7 +204,7 @@ def bench_single(args, mf):
     if not args.no_variants:
         # thread-centric scanning leaves one lane per hub vertex on power-law graphs (minutes
         # per solve at R-MAT scale 22), so only the vertex-centric variants run there
-        algos = ("vc",) if W["algo"] == "vc" else ("tc", "vc")
+        algos = W.get("variants", ("vc",) if W["algo"] == "vc" else ("tc", "vc"))
         for va in algos:
             for vl in ("rcsr", "bcsr", "bcsr2"):
                 vg = ctx.upload(g, mf.LAYOUTS[vl])

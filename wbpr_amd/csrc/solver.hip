@@ -108,14 +108,14 @@ __device__ __forceinline__ void scan_seg(const L &g, const int *h, int u, int se
   }
 }
 
-// All slots of u; deg = their number.
+// All slots of u; deg = their number.  r0 = u's first segment, read by the caller.
 template <int B, class L>
-__device__ __forceinline__ void scan_slots(const L &g, const int *h, int u, int first, int step,
-                                           Cand &c, int &deg) {
+__device__ __forceinline__ void scan_slots(const L &g, const int *h, int u, int2 r0, int first,
+                                           int step, Cand &c, int &deg) {
   deg = 0;
 #pragma unroll
   for (int seg = 0; seg < L::kSeg; ++seg) {
-    const int2 r = g.range(u, seg);
+    const int2 r = seg == 0 ? r0 : g.range(u, seg);
     deg += r.y - r.x;
     scan_seg<B>(g, h, u, seg, r.x, r.y, first, step, c);
   }
@@ -170,22 +170,6 @@ struct Plan {
   int d;       // this lane's share
 };
 
-// One discharge step, finished: relabel (if no neighbour is lower), then every lane whose own
-// candidate arc is admissible at the new height pushes its share of the excess (the shares are
-// a prefix split of e(u) over the lanes' residuals, so the lane holding the minimum arc is
-// always among them).  Relabelling and pushing in one step is the lock-free relabel followed by
-// the push it enables, with the same reads.  Then u and the push targets are appended to the
-// next worklist.  Exactly-once: a vertex is in the current list iff its excess became positive
-// since it was last processed.  The owner re-adds u iff its own atomicSub leaves a positive
-// remainder (or it relabelled below H); a pusher adds w iff its atomicAdd found e(w) == 0.
-// Whichever of two atomics on one excess comes second sees the other's effect, so exactly one
-// of them appends.
-//
-// Push forwarding: the pusher that found e(w) == 0 holds w's only ticket into the next
-// worklist, and no one else is discharging w (a vertex queued for this cycle keeps positive
-// excess until its own step).  Instead of queueing w it may discharge w itself, now: the
-// returned w (or -1) is what the caller continues with.  fcls: forwarding allowed into vertices
-// of degree class <= fcls (-1: no forwarding), so a hub never lands on a narrow tile.
 // This lane pushes d along its candidate arc; true when the push gave w its first excess (w
 // must then be queued, unless it is a sink).
 template <class L>
@@ -206,11 +190,28 @@ __device__ __forceinline__ void queue_next(const CycleArgs &a, Stage *S, int ite
   else append(a.out, item, want, cls, a.nclass);
 }
 
+// One discharge step of a narrow tile, finished: relabel (if no neighbour is lower), then the
+// lane holding the lowest arc pushes its share p.d along it.  Relabelling and pushing in one step
+// is the lock-free relabel followed by the push it enables, with the same reads.  Then u and the
+// push target are appended to the next worklist.  Exactly-once: a vertex is in the current list iff its excess became positive
+// since it was last processed.  The owner re-adds u iff its own atomicSub leaves a positive
+// remainder (or it relabelled below H); a pusher adds w iff its atomicAdd found e(w) == 0.
+// Whichever of two atomics on one excess comes second sees the other's effect, so exactly one
+// of them appends.
+//
+// Push forwarding: the pusher that found e(w) == 0 holds w's only ticket into the next
+// worklist, and no one else is discharging w (a vertex queued for this cycle keeps positive
+// excess until its own step).  Instead of queueing w it may discharge w itself, now: the
+// returned w (or -1) is what the caller continues with.  fcls: forwarding allowed into vertices
+// of degree class <= fcls (-1: no forwarding), so a hub never lands on a narrow tile.
 template <class L>
 __device__ __forceinline__ int finish_vertex(const L &g, const CycleArgs &a, Stage *S, int u,
                                              int hu, const Plan &p, const Cand &mine,
                                              bool leader, int deg, int fcls, Stats &st) {
   bool want_u = false, want_w = false;
+  // the classes the appends need, read while the atomics below are in flight
+  const int cu = (leader && a.vcls) ? a.vcls[u] : 0;
+  const int cw = (p.d > 0 && a.vcls) ? a.vcls[mine.w] : 0;
   if (leader) {
     st.active++;
     st.slots += deg;
@@ -225,23 +226,22 @@ __device__ __forceinline__ int finish_vertex(const L &g, const CycleArgs &a, Sta
       a.h[u] = p.hnew;
       st.relabels++;
     }
+    // the owner's subtraction and the push touch different vertices: both are in flight
+    // together (the owner is the pusher when the tile has one lane)
+    int old_u = 0;
+    if (leader) old_u = atomicSub(&a.e[u], p.total);
     want_w = push_arc(g, a, mine, p.d, st);
-    if (leader) {
-      const int old_u = atomicSub(&a.e[u], p.total);
-      want_u = old_u - p.total > 0;
-    }
+    want_u = leader && old_u - p.total > 0;
   }
   // forward only into a vertex not yet discharged in this cycle: a second discharge in one
   // launch could read residuals and excess from cache lines older than the first one's atomics
   // (too high), where a first discharge can only see values that are too low
   int fwd = -1;
-  if (want_w && fcls >= 0 && (!a.vcls || a.vcls[mine.w] <= fcls) &&
+  if (want_w && fcls >= 0 && cw <= fcls &&
       atomicExch(&a.stamp[mine.w], cycle_now(a)) != cycle_now(a)) {
     fwd = mine.w;
     want_w = false;
   }
-  const int cu = a.vcls ? a.vcls[u] : 0;
-  const int cw = (want_w && a.vcls) ? a.vcls[mine.w] : 0;
   if (S) {
     staged_append(*S, a.out, u, want_u, cu, a.nclass);
     staged_append(*S, a.out, mine.w, want_w, cw, a.nclass);
@@ -261,17 +261,25 @@ __device__ __forceinline__ void plan_heights(unsigned long long best, int hu, in
 // One discharge step of u by a tile of T lanes (T < kMultiPushLanes, inside one wave): the
 // lowest residual arc gets min(e(u), r) after a relabel to its height + 1 if it is not lower.
 // Returns the vertex the tile continues with (push forwarding, see finish_vertex) or -1; fcls is
-// the forwarding class limit (-1: none).
+// the forwarding class limit (-1: none).  claim: u comes from the worklist and forwarding is on,
+// so the tile first stamps it (claim_queued); the stamp's exchange is in flight together with
+// the reads of u's height, excess and slot range, which a failed claim simply discards (a first
+// discharge of u in this launch reads the same values before or after its claim).
 template <int T, class L>
 __device__ __forceinline__ int discharge_tile(const L &g, const CycleArgs &a, Stage *S, int u,
-                                              int lane, int fcls, Stats &st) {
+                                              int lane, int fcls, bool claim, Stats &st) {
+  int old = 0;
+  if (claim && lane == 0) old = atomicExch(&a.stamp[u], cycle_now(a));
   const int hu = a.h[u];
+  const int e0 = lane == 0 ? a.e[u] : 0;
+  const int2 r0 = g.range(u, 0);
+  if (claim && __shfl(old, 0, T) == cycle_now(a)) return -1;  // discharged in this cycle already
   // e(u) grows while the tile runs: every lane uses lane 0's read (the value subtracted below)
-  const int eu = __shfl(a.e[u], 0, T);
+  const int eu = __shfl(e0, 0, T);
   if (hu >= a.H) return -1;  // became inactive after it was queued (all lanes read the same value)
   Cand c;
   int deg;
-  scan_slots<scan_batch<T>()>(g, a.h, u, lane, T, c, deg);
+  scan_slots<scan_batch<T>()>(g, a.h, u, r0, lane, T, c, deg);
   Plan p;
   const unsigned long long best = tile_min<T>(c.best);
   plan_heights(best, hu, a.H, p);
@@ -447,14 +455,27 @@ __device__ __forceinline__ void discharge_multi(const L &g, const CycleArgs &a, 
 }
 
 // One discharge step of u by T lanes; returns the vertex to continue with (forwarding) or -1.
+// The tile that takes a queued vertex stamps it with the cycle before its step, so no forwarder
+// discharges it a second time in the same cycle (finish_vertex).  The stamp's old value is
+// consumed, which orders the exchange before the step's own atomics.
+template <int T>
+__device__ __forceinline__ bool claim_queued(const CycleArgs &a, int u, int lane) {
+  int old = 0;
+  const int cyc = cycle_now(a);
+  if (lane == 0) old = atomicExch(&a.stamp[u], cyc);
+  return __shfl(old, 0, T) != cyc;
+}
+
+// claim: u was taken from the worklist while forwarding is on (stamp it first, claim_queued).
 template <int T, class L>
 __device__ __forceinline__ int discharge(const L &g, const CycleArgs &a, Stage *S, int u,
-                                         int lane, int fcls, Stats &st) {
+                                         int lane, int fcls, bool claim, Stats &st) {
   if constexpr (T >= kMultiPushLanes) {
+    if (claim && !claim_queued<T>(a, u, lane)) return -1;
     discharge_multi<T>(g, a, S, u, lane, st);
     return -1;  // several lanes push: no single vertex to continue with
   } else {
-    return discharge_tile<T>(g, a, S, u, lane, fcls, st);
+    return discharge_tile<T>(g, a, S, u, lane, fcls, claim, st);
   }
 }
 
@@ -649,17 +670,6 @@ __device__ __forceinline__ void cycle_giant(const L &g, const CycleArgs &a, Stag
   }
 }
 
-// The tile that takes a queued vertex stamps it with the cycle before its step, so no forwarder
-// discharges it a second time in the same cycle (finish_vertex).  The stamp's old value is
-// consumed, which orders the exchange before the step's own atomics.
-template <int T>
-__device__ __forceinline__ bool claim_queued(const CycleArgs &a, int u, int lane) {
-  int old = 0;
-  const int cyc = cycle_now(a);
-  if (lane == 0) old = atomicExch(&a.stamp[u], cyc);
-  return __shfl(old, 0, T) != cyc;
-}
-
 template <int T, class L>
 __device__ __forceinline__ void cycle_class(const L &g, const CycleArgs &a, Stage *S, int c,
                                             int cnt, int b, int nblk, Stats &st) {
@@ -671,9 +681,8 @@ __device__ __forceinline__ void cycle_class(const L &g, const CycleArgs &a, Stag
     const int lane = (int)threadIdx.x % T;
     for (int i = tile; i < cnt; i += ntiles) {
       int v = a.in.items[base + i];
-      if (a.fwd_hops > 0 && !claim_queued<T>(a, v, lane)) continue;
       for (int hop = 0; v >= 0; ++hop)
-        v = discharge<T>(g, a, S, v, lane, hop < a.fwd_hops ? c : -1, st);
+        v = discharge<T>(g, a, S, v, lane, hop < a.fwd_hops ? c : -1, hop == 0 && a.fwd_hops > 0, st);
     }
   } else {
     for (int i = b; i < cnt; i += nblk) discharge_multi<kBlock>(g, a, S, a.in.items[base + i], (int)threadIdx.x, st);
@@ -704,13 +713,17 @@ __global__ __launch_bounds__(kBlock) void k_cycle(L g, CycleArgs a, Dispatch d) 
   const long long t_start = a.work ? clock64() : 0;
   if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses)
     a.clear_cnt[threadIdx.x * kCtlStride] = 0;
-  __shared__ Stage S;
-  stage_init(S);
   const BlockSlot bs = block_slot(d);
   const int c = bs.c, b = bs.b, nblk = bs.nblk;
   const int cnt = list_count(a.in, c);
+  // the grid is sized for the class populations; a block whose first tile has no item leaves
+  // at once (small worklists: most of the grid)
+  const int lanes = pick(d.lanes, c);
+  if (lanes <= 64 ? b * (kBlock / lanes) >= cnt : (lanes == kBlock ? b >= cnt : cnt == 0)) return;
+  __shared__ Stage S;
+  stage_init(S);
   Stats st{0, 0, 0, 0};
-  switch (pick(d.lanes, c)) {
+  switch (lanes) {
     case 1: cycle_class<1>(g, a, &S, c, cnt, b, nblk, st); break;
     case 2: cycle_class<2>(g, a, &S, c, cnt, b, nblk, st); break;
     case 4: cycle_class<4>(g, a, &S, c, cnt, b, nblk, st); break;
@@ -922,9 +935,6 @@ template <class L>
 __global__ __launch_bounds__(kBlock) void k_bfs_level(L g, BfsArgs a, Dispatch d, int level) {
   if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses)
     pick(a.fcnt, (level + 2) % 3)[threadIdx.x * kCtlStride] = 0;
-  __shared__ Stage sf, sa;
-  stage_init(sf);
-  stage_init(sa);
   const BlockSlot bs = block_slot(d);
   const int c = bs.c, b = bs.b, nblk = bs.nblk;
   // the two frontier lists share their class regions; only the item arrays alternate (selecting
@@ -935,6 +945,14 @@ __global__ __launch_bounds__(kBlock) void k_bfs_level(L g, BfsArgs a, Dispatch d
   in.cnt = pick(a.fcnt, level % 3);
   out.cnt = pick(a.fcnt, (level + 1) % 3);
   const int cnt = list_count(in, c);
+  // blocks whose first tile has no frontier vertex leave at once (tiles of lanes / 4 BFS lanes;
+  // hub classes share their vertices among all their blocks)
+  const int lanes = pick(d.lanes, c);
+  const int tt = lanes == 1 ? 1 : ((lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64) ? lanes / 4 : 0);
+  if (cnt == 0 || (tt > 0 && b * (kBlock / tt) >= cnt)) return;
+  __shared__ Stage sf, sa;
+  stage_init(sf);
+  stage_init(sa);
   long long nv = 0, ns = 0;
   bfs_dispatch<false>(g, a, in, out, BfsStages{&sf, &sa}, c, pick(d.lanes, c), cnt,
                b * kBlock + (int)threadIdx.x, nblk * kBlock, level, nv, ns);
@@ -1427,9 +1445,9 @@ __device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &c
   case TT:                                                                         \
     for (int i = tid / TT; i < cnt; i += kBlock / TT) {                             \
       int v = ca.in.items[base + i];                                               \
-      if (ca.fwd_hops > 0 && !claim_queued<TT>(ca, v, tid % TT)) continue;         \
       for (int hop = 0; v >= 0; ++hop)                                             \
-        v = discharge<TT>(g, ca, nullptr, v, tid % TT, hop < ca.fwd_hops ? c : -1, st); \
+        v = discharge<TT>(g, ca, nullptr, v, tid % TT, hop < ca.fwd_hops ? c : -1,  \
+                          hop == 0 && ca.fwd_hops > 0, st);                        \
     }                                                                              \
     break;
     MF_BATCH_CASE(1)
