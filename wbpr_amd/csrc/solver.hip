@@ -1699,7 +1699,7 @@ constexpr int kGraphChunkMax = 60;           // longest captured chunk
 constexpr int64_t kBottomUpAlpha = 14;
 constexpr int64_t kBottomUpBeta = 24;
 constexpr int64_t kBottomUpMinVertices = 1 << 16;
-constexpr int kBatchGroupsPerCU = 4;         // resident workgroups per CU in the batched solve
+constexpr int kBatchGroupsPerCU = 4;         // at most this many batched workgroups per CU
 
 template <class T>
 static mf_status dmalloc(T **p, size_t count) {
@@ -2566,7 +2566,14 @@ static mf_status run_batch_wg(mf_context *ctx, DevGraph &G, const int32_t *vbase
   b.fwd_hops = o.forward_hops < 0 ? 0 : (o.forward_hops > 0 ? o.forward_hops : kDefaultForwardHops);
   b.stamp = G.stamp;
   if (o.cycles_per_relabel > 0) b.ratio = 1e30f;  // fixed cadence: only the cycle cap ends a round
-  const int nwg = std::max(1, std::min(k, kBatchGroupsPerCU * ctx->num_cu));
+  // persistent workgroups: as many as can be resident at once (the kernel's registers decide),
+  // so none waits for another to finish before it starts pulling instances
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_wg<LayoutBCSR>, kBlock, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  const int nwg = std::max(1, std::min(k, std::min(per_cu, kBatchGroupsPerCU) * ctx->num_cu));
   const double t0 = now_ms();
   const int rgrid = std::max(1, (int)std::min<int64_t>((G.S + 255) / 256, 16 * ctx->num_cu));
   k_reset_pairs<<<rgrid, 256, 0, st>>>(G.dc, G.cap0, G.S, 1);
