@@ -1102,6 +1102,7 @@ struct WgOut {
   int *u, *b, *e;  // next buffer's entries (b, e: slot ranges, one-segment layouts only)
   int *n;          // entries per group (may count past the capacity: readers clamp)
   ListSet dev;     // device list of the next level (items in global memory, counters in LDS)
+  ListSet act;     // the cycle worklist (items in global memory, counters in LDS for the launch)
   int *ovf;        // set when any vertex went to the device list
   int nclass;
 };
@@ -1191,7 +1192,7 @@ __device__ __forceinline__ void wg_expand(const L &g, const BfsArgs &a, const Wg
 #pragma unroll
       for (int k = 0; k < B; ++k) {
         wg_append<kPre>(o, w[k], claim[k], cw[k], wb[k], we[k]);
-        append(a.active, w[k], claim[k] && ew[k] > 0, aw[k], a.nclass_act);
+        append(o.act, w[k], claim[k] && ew[k] > 0, aw[k], a.nclass_act);
       }
     }
   }
@@ -1206,6 +1207,7 @@ __global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d,
   __shared__ int s_e[kPre ? 2 : 1][kPre ? kWgCap : 1];
   __shared__ int s_n[2][3];
   __shared__ int s_ocnt[kNumClasses];
+  __shared__ int s_acnt[kNumClasses];  // the cycle worklist's counters while this kernel runs
   __shared__ int s_ovf, s_go, s_size;
   __shared__ long long s_est;
   const int tid = (int)threadIdx.x;
@@ -1219,6 +1221,9 @@ __global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d,
     s_n[0][tid] = 0;
     s_n[1][tid] = 0;
   }
+  // nothing else appends to the cycle worklist while this kernel runs: its counters move to LDS
+  // (no returning global atomic per appending wave) and back at the end
+  if (tid < kNumClasses) s_acnt[tid] = a.active.cnt[tid * a.active.cstride];
   if (tid == 0) {
     int n[3] = {0, 0, 0}, sz = 0;
     long long slots = 0;  // estimated from the class means: hubs make a few vertices heavy
@@ -1275,6 +1280,9 @@ __global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d,
     o.dev.items = (level & 1) ? a.fr[0].items : a.fr[1].items;  // the next level's item array
     o.dev.cnt = s_ocnt;
     o.dev.cstride = 1;
+    o.act = a.active;
+    o.act.cnt = s_acnt;
+    o.act.cstride = 1;
     o.ovf = &s_ovf;
     o.nclass = a.nclass;
     long long est = 0;
@@ -1331,6 +1339,7 @@ __global__ __launch_bounds__(kWgBlock) void k_bfs_wg(L g, BfsArgs a, Dispatch d,
   __syncthreads();
   if (tid < kNumClasses) {
     const int c = tid;
+    a.active.cnt[c * a.active.cstride] = s_acnt[c];
     pick(a.fcnt, level % 3)[c * kCtlStride] = s_ocnt[c];
     pick(a.fcnt, (level + 1) % 3)[c * kCtlStride] = 0;
     pick(a.fcnt, (level + 2) % 3)[c * kCtlStride] = 0;
