@@ -216,6 +216,46 @@ def test_giant_vertices_fixed_cadence(mf, orc, ctx):
         solve_and_certify(mf, orc, ctx, g, "vc", "bcsr", expect, cycles_per_relabel=k)
 
 
+def layered_graph(mf, seed, layers=8, wide=3000, narrow=400):
+    """Layers alternating between `wide` low-degree vertices and `narrow` vertices of 30-40 slots:
+    BFS frontiers the one-workgroup kernel keeps (at most 4,096 vertices, few slots) but whose
+    LDS groups overflow (2,048 low-degree, 256 mid-degree entries), so vertices spill to the
+    device lists and the kernel hands over mid-relabel."""
+    rng = np.random.default_rng(seed)
+    sizes = [wide if i % 2 == 0 else narrow for i in range(layers)]
+    base = np.cumsum([2] + sizes)
+    s, t, n = 0, 1, int(base[-1])
+    u, v = [], []
+    u.append(np.zeros(sizes[0], dtype=np.int64))
+    v.append(np.arange(sizes[0]) + base[0])
+    for i in range(layers - 1):
+        k = 4 if sizes[i] == wide else 8
+        src = np.repeat(np.arange(sizes[i]) + base[i], k)
+        u.append(src)
+        v.append(rng.integers(0, sizes[i + 1], size=src.size) + base[i + 1])
+    u.append(np.arange(sizes[-1]) + base[layers - 1])
+    v.append(np.full(sizes[-1], t))
+    u = np.concatenate(u)
+    v = np.concatenate(v)
+    key = u * n + v  # drop parallel arcs
+    _, first = np.unique(key, return_index=True)
+    u, v = u[np.sort(first)], v[np.sort(first)]
+    cap = rng.integers(1, 100, size=u.size)
+    return mf.Graph.from_arcs(n, u, v, cap, s, t)
+
+
+@pytest.mark.parametrize("algo,layout", [("vc", "bcsr"), ("tc", "rcsr"), ("vc", "bcsr2")])
+def test_bfs_lds_groups_spill(mf, orc, ctx, algo, layout):
+    # one-workgroup BFS: frontiers beyond an LDS group's capacity spill to the device lists.
+    # Narrow layers of 400 overflow the mid-degree group (256); narrow layers of 200 fit it, and
+    # expanding one claims nearly all 3,000 vertices of the wide layer before it, overflowing the
+    # low-degree group (2,048).  Either way the kernel hands the frontier to grid-wide levels.
+    for seed, narrow in ((1, 400), (2, 200)):
+        g = layered_graph(mf, seed, narrow=narrow)
+        r = solve_and_certify(mf, orc, ctx, g, algo, layout, orc.dinic_maxflow(g))
+        assert r.bfs_grid_levels > 0
+
+
 def test_rmat22_full_size(mf, orc, ctx, golden):
     g, value = generated_graph(mf, golden, "rmat22_s1")
     solve_and_certify(mf, orc, ctx, g, "vc", "bcsr", value)
