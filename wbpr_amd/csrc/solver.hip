@@ -1092,7 +1092,7 @@ __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(L g, BfsArgs a, Dispat
 constexpr int kWgCapA = 2048, kWgCapB = 256, kWgCapC = 32;  // LDS frontier entries per group
 constexpr int kWgCap = kWgCapA + kWgCapB + kWgCapC;
 
-__device__ __forceinline__ int wg_group(int cls) { return cls <= 1 ? 0 : (cls <= 3 ? 1 : 2); }
+__host__ __device__ constexpr int wg_group(int cls) { return cls <= 1 ? 0 : (cls <= 3 ? 1 : 2); }
 __device__ __forceinline__ int wg_cap(int q) { return q == 0 ? kWgCapA : (q == 1 ? kWgCapB : kWgCapC); }
 __device__ __forceinline__ int wg_off(int q) { return q == 0 ? 0 : (q == 1 ? kWgCapA : kWgCapA + kWgCapB); }
 
@@ -2266,6 +2266,13 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
         sl += (double)hstate[ctl_cnt(CTL_FCNT, lv % 3, c)] * ba.cls_mean[c];
       return sl;
     };
+    // the frontier fits the one-workgroup kernel's LDS groups (else it would return at once)
+    auto fits_wg = [&](int lv) {
+      int grp[3] = {0, 0, 0};
+      for (int c = 0; c < bcfg.nclass; ++c)
+        grp[wg_group(c)] += hstate[ctl_cnt(CTL_FCNT, lv % 3, c)];
+      return grp[0] <= kWgCapA && grp[1] <= kWgCapB && grp[2] <= kWgCapC;
+    };
     for (;;) {
       k_bfs_wg<L><<<1, kWgBlock, 0, st>>>(lay, ba, bdisp, level, fmax);
       HIP_TRY(hipGetLastError());
@@ -2298,7 +2305,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
         HIP_TRY(hipStreamSynchronize(st));
         size = 0;
         for (int c = 0; c < bcfg.nclass; ++c) size += hstate[ctl_cnt(CTL_FCNT, level % 3, c)];
-        if (size <= fmax_low && frontier_slots(level) <= ba.wg_slots / 2) break;
+        if (size <= fmax_low && frontier_slots(level) <= ba.wg_slots / 2 && fits_wg(level)) break;
       }
       if (size == 0) break;
     }
