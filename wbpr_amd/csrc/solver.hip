@@ -29,6 +29,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #include "build.h"
@@ -913,7 +914,13 @@ __global__ void k_bfs_init(BfsArgs a, int *ccnt0, int *ccnt1) {
     a.fcnt[2][c * kCtlStride] = 0;
     ccnt0[c * kCtlStride] = 0;
     ccnt1[c * kCtlStride] = 0;
-    if (c == 0) a.flags[0] = 0;
+    if (c == 0) {
+      a.flags[0] = 0;
+      a.state[2] = 0;  // grid levels' marks (k_bfs_level)
+      a.state[3] = 0;
+      a.state[4] = 0;  // top-down slots explored (a 64-bit counter in words 4 and 5)
+      a.state[5] = 0;
+    }
   }
 }
 
@@ -930,9 +937,62 @@ __global__ void k_bfs_seed(BfsArgs a, const int *src, const int *snk, int k) {
   }
 }
 
+// LDS frontier entries per group of the one-workgroup BFS (below), by degree class
+constexpr int kWgCapA = 2048, kWgCapB = 256, kWgCapC = 32;
+constexpr int kWgCap = kWgCapA + kWgCapB + kWgCapC;
+__host__ __device__ constexpr int wg_group(int cls) { return cls <= 1 ? 0 : (cls <= 3 ? 1 : 2); }
+
+// Where a batch of top-down levels stops: a level after the batch's first does nothing when its
+// frontier would rather go bottom-up (estimated slots above stop_slots) or back to the
+// one-workgroup kernel (at most small vertices and half its slot limit, fitting its LDS groups),
+// or is empty.  Every block decides alike from the same complete counters.  A level that runs
+// marks the next one (state[2 + (next & 1)] = next, a word it does not read itself); a level
+// runs only if it was marked, so once one level of the batch does nothing, none after it runs
+// (the counters are triple-buffered: three levels on, a skipped level's input counters would
+// look valid again).
+struct LevelStop {
+  int first;             // the batch's first level (always runs)
+  long long stop_slots;  // bottom-up switch point in estimated frontier slots
+  int small;             // hand-back size to the one-workgroup kernel
+};
+
+template <class L>
+__device__ __forceinline__ bool level_runs(const BfsArgs &a, const Dispatch &d, const ListSet &in,
+                                           int level, const LevelStop &ls, long long &slots) {
+  int size = 0, ga = 0, gb = 0, gc = 0;  // frontier size and its one-workgroup LDS groups
+  slots = 0;
+  for (int c = 0; c < d.nclass; ++c) {
+    const int k = list_count(in, c);
+    size += k;
+    slots += (long long)k * pick(a.cls_mean, c);
+    const int q = wg_group(c);
+    ga += q == 0 ? k : 0;
+    gb += q == 1 ? k : 0;
+    gc += q == 2 ? k : 0;
+  }
+  const bool fits = ga <= kWgCapA && gb <= kWgCapB && gc <= kWgCapC;
+  if (level == ls.first) return true;
+  if (a.state[2 + (level & 1)] != level) return false;  // the level before did not run
+  if (size == 0 || slots > ls.stop_slots) return false;
+  if (size <= ls.small && slots <= a.wg_slots / 2 && fits) return false;
+  return true;
+}
+
 // One BFS level over the whole grid.
 template <class L>
-__global__ __launch_bounds__(kBlock) void k_bfs_level(L g, BfsArgs a, Dispatch d, int level) {
+__global__ __launch_bounds__(kBlock) void k_bfs_level(L g, BfsArgs a, Dispatch d, int level,
+                                                      LevelStop ls) {
+  {
+    ListSet cur = a.fr[0];
+    cur.cnt = pick(a.fcnt, level % 3);
+    long long slots;
+    if (!level_runs<L>(a, d, cur, level, ls, slots)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.state[2 + ((level + 1) & 1)] = level + 1;  // the next level may run
+      // estimated slots of the top-down frontiers expanded so far (the bottom-up rule's input)
+      atomicAdd((unsigned long long *)(a.state + 4), (unsigned long long)slots);
+    }
+  }
   if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses)
     pick(a.fcnt, (level + 2) % 3)[threadIdx.x * kCtlStride] = 0;
   const BlockSlot bs = block_slot(d);
@@ -1089,10 +1149,6 @@ __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(L g, BfsArgs a, Dispat
 // returns.  No device-scope fence is needed: every value this kernel reads that another kernel
 // wrote was written before the launch.
 
-constexpr int kWgCapA = 2048, kWgCapB = 256, kWgCapC = 32;  // LDS frontier entries per group
-constexpr int kWgCap = kWgCapA + kWgCapB + kWgCapC;
-
-__host__ __device__ constexpr int wg_group(int cls) { return cls <= 1 ? 0 : (cls <= 3 ? 1 : 2); }
 __device__ __forceinline__ int wg_cap(int q) { return q == 0 ? kWgCapA : (q == 1 ? kWgCapB : kWgCapC); }
 __device__ __forceinline__ int wg_off(int q) { return q == 0 ? 0 : (q == 1 ? kWgCapA : kWgCapA + kWgCapB); }
 
@@ -1697,6 +1753,7 @@ constexpr float kDefaultRelabelRatio = 2.0f;
 constexpr int kMaxAdaptiveCycles = 1 << 14;  // cap of cycles per round in adaptive mode
 constexpr int kDefaultBfsFrontier = 4096;    // one-workgroup BFS up to this frontier size
 constexpr int kWgBfsSlots = 32768;           // ... and up to about this many frontier slots
+constexpr int kGridLevelBatch = 4;           // top-down grid levels issued per read-back
 constexpr int kDefaultForwardHops = 2;       // push forwarding: vertices a tile chains per cycle
 constexpr int kCycleChunk = 64;              // most cycles launched between two worklist checks
 constexpr int kFirstChunk = 8;               // first chunk, before a cycle has been timed
@@ -1875,7 +1932,9 @@ enum {
   CTL_FCNT = 0,                                        // 3 x kNumClasses frontier counters
   CTL_CCNT = CTL_FCNT + 3 * kNumClasses * kCtlStride,  // 3 x kNumClasses worklist counters
   CTL_FLAGS = CTL_CCNT + 3 * kNumClasses * kCtlStride, // [0] source reached, [1] list overflow
-  CTL_STATE = CTL_FLAGS + kCtlStride,                  // [0] level, [1] frontier size
+  CTL_STATE = CTL_FLAGS + kCtlStride,                  // [0] level, [1] frontier size,
+                                                       // [2..3] grid levels' marks,
+                                                       // [4..5] slots explored
   CTL_SIZE = CTL_STATE + kCtlStride
 };
 
@@ -2282,27 +2341,45 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       int size = hstate[CTL_STATE + 1];
       if (size == 0) break;
       bool bottom_up = false;
-      for (;;) {  // frontier too large for one workgroup: one launch per level
+      for (;;) {  // frontier too large for one workgroup: grid launches per level
         // frontier slots estimated from its class counts and the classes' mean degrees
         const double mf = frontier_slots(level);
+        long long td_explored;
+        std::memcpy(&td_explored, &hstate[CTL_STATE + 4], sizeof(td_explored));
+        const double mu = std::max(0.0, (double)G.S - explored - (double)td_explored);
         if (bu_allowed) {
-          const double mu = std::max(0.0, (double)G.S - explored);
           if (!bottom_up && mf * kBottomUpAlpha > mu) bottom_up = true;
           else if (bottom_up && (int64_t)size * kBottomUpBeta < G.nonzero) bottom_up = false;
-          explored += mf;
         }
         bottom_up = bottom_up || bu_always;
+        int next;
         if (bottom_up) {
           k_bfs_bottom_up<L><<<bugrid, kBlock, 0, st>>>(lay, ba, budisp, G.vo, level);
           ++bfs_bu_levels;
+          explored += mf;
+          next = level + 1;
         } else {
-          k_bfs_level<L><<<bgrid, kBlock, 0, st>>>(lay, ba, bdisp, level);
+          // top-down levels go kGridLevelBatch at a time between two read-backs; the levels after
+          // the first stop by themselves (LevelStop) when the frontier empties, nears half the
+          // bottom-up switch point or fits the one-workgroup kernel again, so the host decides
+          // those turns as before, at most a batch later, and a read-back (a host round trip)
+          // covers several levels of a long-diameter graph
+          LevelStop ls;
+          ls.first = level;
+          ls.small = fmax_low;
+          ls.stop_slots = (bu_allowed && !bu_always)
+                              ? (long long)(mu / kBottomUpAlpha / 2)
+                              : std::numeric_limits<long long>::max();
+          for (int q = 0; q < kGridLevelBatch; ++q)
+            k_bfs_level<L><<<bgrid, kBlock, 0, st>>>(lay, ba, bdisp, level + q, ls);
+          next = -1;
         }
         HIP_TRY(hipGetLastError());
-        ++level;
-        ++bfs_grid_levels;
         HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_SIZE, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
+        if (next < 0) next = std::max(hstate[CTL_STATE + 2], hstate[CTL_STATE + 3]);
+        bfs_grid_levels += next - level;
+        level = next;
         size = 0;
         for (int c = 0; c < bcfg.nclass; ++c) size += hstate[ctl_cnt(CTL_FCNT, level % 3, c)];
         if (size <= fmax_low && frontier_slots(level) <= ba.wg_slots / 2 && fits_wg(level)) break;
