@@ -1,5 +1,7 @@
 // Autotuner of the solver's launch parameters: trial solves over a small candidate set, by
-// coordinate descent (one parameter at a time, keeping the best so far).
+// coordinate descent (one parameter at a time, keeping the best so far).  The parameters
+// interact (tile widths, BFS hand-over and graph replay most), so the descent sweeps them all
+// again while a sweep still changes the options, at most kPasses times.
 //
 //   tile widths       mf_options.class_lanes: lanes per vertex of each VC degree class.  Narrow
 //                     tiles put more vertices in flight per wave, wide tiles more slots per
@@ -16,7 +18,8 @@
 //
 // Lock-free schedules differ from run to run (solve times spread by +-15% on long-path graphs),
 // so a candidate's time is the mean of three solves, and the winner is checked at the end: it
-// and the starting options are timed again over four solves each, and the faster of the two is
+// and the starting options are timed again over six solves each, alternating between the two so
+// that a drift of solve times during the run favours neither, and the faster of the two is
 // returned -- tuning never hands back options slower than those it started from.  Every trial
 // must give the same max-flow value (the value is unique); a mismatch is reported as
 // MF_ERR_INTERNAL.
@@ -30,7 +33,8 @@
 namespace {
 
 constexpr int kRepeats = 3;   // solves per candidate (mean)
-constexpr int kValidate = 4;  // solves per option set in the final check
+constexpr int kValidate = 6;  // solves per option set in the final check (alternating)
+constexpr int kPasses = 3;    // sweeps over all parameters, while a sweep changes something
 
 // VC tile-width presets (lanes for degree classes <=8, <=16, <=32, <=1024, larger)
 const uint16_t kLanePresets[][5] = {
@@ -60,31 +64,11 @@ bool same_lanes(const uint16_t *a, const uint16_t *b) {
   return true;
 }
 
-// Coordinate descent over the candidates; run(opts, ms) performs one timed trial.
-template <class Run>
-mf_status descend(mf_options cur, bool tune_lanes, bool tune_batch_algo, Run run,
-                  mf_options *best, double *best_ms) {
-  // the result names every tuned value explicitly (0 would mean "library default")
-  cur.relabel_time_ratio = effective_ratio(cur);
-  cur.bfs_block_frontier = effective_frontier(cur);
-  if (cur.forward_hops == 0) cur.forward_hops = 2;
-  if (tune_lanes)
-    for (int k = 0; k < 5; ++k)
-      if (!cur.class_lanes[k]) cur.class_lanes[k] = kLanePresets[0][k];
-  double cur_ms = 0;
-  mf_status st = run(cur, cur_ms, kRepeats);
-  if (st != MF_OK) return st;
-  const mf_options start = cur;
-  auto consider = [&](const mf_options &t) -> mf_status {
-    double ms = 0;
-    mf_status s = run(t, ms, kRepeats);
-    if (s != MF_OK) return s;
-    if (ms < cur_ms) {
-      cur = t;
-      cur_ms = ms;
-    }
-    return MF_OK;
-  };
+// One sweep over every parameter: each candidate value is tried with the others as they stand
+// in cur, which consider(t) replaces by t when t is faster (cur is the caller's live value).
+template <class Consider>
+mf_status sweep(const mf_options &cur, bool tune_lanes, bool tune_batch_algo, Consider consider) {
+  mf_status st = MF_OK;
   if (tune_batch_algo) {
     mf_options t = cur;
     t.batch_algo = cur.batch_algo == 0 ? 1 : 0;
@@ -126,10 +110,50 @@ mf_status descend(mf_options cur, bool tune_lanes, bool tune_batch_algo, Run run
       if ((st = consider(t)) != MF_OK) return st;
     }
   }
+  return MF_OK;
+}
+
+// Coordinate descent over the candidates; run(opts, ms, reps) times reps solves (mean).
+template <class Run>
+mf_status descend(mf_options cur, bool tune_lanes, bool tune_batch_algo, Run run,
+                  mf_options *best, double *best_ms) {
+  // the result names every tuned value explicitly (0 would mean "library default")
+  cur.relabel_time_ratio = effective_ratio(cur);
+  cur.bfs_block_frontier = effective_frontier(cur);
+  if (cur.forward_hops == 0) cur.forward_hops = 2;
+  if (tune_lanes)
+    for (int k = 0; k < 5; ++k)
+      if (!cur.class_lanes[k]) cur.class_lanes[k] = kLanePresets[0][k];
+  double cur_ms = 0;
+  mf_status st = run(cur, cur_ms, kRepeats);
+  if (st != MF_OK) return st;
+  const mf_options start = cur;
+  bool changed = false;
+  auto consider = [&](const mf_options &t) -> mf_status {
+    double ms = 0;
+    const mf_status s = run(t, ms, kRepeats);
+    if (s != MF_OK) return s;
+    if (ms < cur_ms) {
+      cur = t;
+      cur_ms = ms;
+      changed = true;
+    }
+    return MF_OK;
+  };
+  for (int pass = 0; pass < kPasses; ++pass) {
+    changed = false;
+    // sweep() reads cur live: each parameter is tried on top of the values chosen before it
+    if ((st = sweep(cur, tune_lanes, tune_batch_algo, consider)) != MF_OK) return st;
+    if (!changed) break;
+  }
   if (std::memcmp(&cur, &start, sizeof(cur)) != 0) {
     double a_ms = 0, b_ms = 0;
-    if ((st = run(start, a_ms, kValidate)) != MF_OK || (st = run(cur, b_ms, kValidate)) != MF_OK)
-      return st;
+    for (int r = 0; r < kValidate; ++r) {
+      double x = 0, y = 0;
+      if ((st = run(start, x, 1)) != MF_OK || (st = run(cur, y, 1)) != MF_OK) return st;
+      a_ms += x / kValidate;
+      b_ms += y / kValidate;
+    }
     if (a_ms <= b_ms) {
       cur = start;
       cur_ms = a_ms;
