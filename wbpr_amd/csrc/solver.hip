@@ -1921,6 +1921,7 @@ struct DevGraph {
   // host copies for result read-back
   std::vector<int32_t> h_off, h_dst, h_cap, h_fslot, h_src, h_snk;
   bool solved = false;
+  bool fresh = false;  // residuals still equal the capacities (built, not solved since)
 
   LayoutBCSR bcsr() const { return LayoutBCSR{off, dc, rev}; }
   LayoutBCSR2 bcsr2() const { return LayoutBCSR2{off, dst, cr}; }
@@ -2117,6 +2118,7 @@ static mf_status upload(mf_context *ctx, const mf_csr *c, const int32_t *src, co
   HIP_TRY(hipMemcpyAsync(G.snk, snk, sizeof(int) * k, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemsetAsync(G.ctl, 0, sizeof(int) * CTL_SIZE, st));
   HIP_TRY(hipStreamSynchronize(st));
+  G.fresh = true;  // the layout build wrote every residual as its capacity
   return MF_OK;
 }
 
@@ -2294,9 +2296,13 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
 
   const double t0 = now_ms();
   const int rgrid = std::max(1, (int)std::min<int64_t>((G.S + 255) / 256, 16 * ctx->num_cu));
-  if (G.layout == MF_LAYOUT_BCSR) k_reset_pairs<<<rgrid, 256, 0, st>>>(G.dc, G.cap0, G.S, 1);
-  else if (G.layout == MF_LAYOUT_BCSR2) k_reset_pairs<<<rgrid, 256, 0, st>>>(G.cr, G.cap0, G.S, 0);
-  else k_reset_pairs<<<rgrid, 256, 0, st>>>(G.fdc, G.cap0, G.m, 1);
+  // residuals back to the capacities, unless no solve has run since the layout was built
+  if (!G.fresh) {
+    if (G.layout == MF_LAYOUT_BCSR) k_reset_pairs<<<rgrid, 256, 0, st>>>(G.dc, G.cap0, G.S, 1);
+    else if (G.layout == MF_LAYOUT_BCSR2) k_reset_pairs<<<rgrid, 256, 0, st>>>(G.cr, G.cap0, G.S, 0);
+    else k_reset_pairs<<<rgrid, 256, 0, st>>>(G.fdc, G.cap0, G.m, 1);
+  }
+  G.fresh = false;
   k_reset_vertices<<<rgrid, 256, 0, st>>>(G.h, G.e, G.stamp, n,
                                           G.layout == MF_LAYOUT_RCSR ? G.bcf : nullptr,
                                           G.m);
@@ -2669,7 +2675,8 @@ static mf_status run_batch_wg(mf_context *ctx, DevGraph &G, const int32_t *vbase
   const int nwg = std::max(1, std::min(k, std::min(per_cu, kBatchGroupsPerCU) * ctx->num_cu));
   const double t0 = now_ms();
   const int rgrid = std::max(1, (int)std::min<int64_t>((G.S + 255) / 256, 16 * ctx->num_cu));
-  k_reset_pairs<<<rgrid, 256, 0, st>>>(G.dc, G.cap0, G.S, 1);
+  if (!G.fresh) k_reset_pairs<<<rgrid, 256, 0, st>>>(G.dc, G.cap0, G.S, 1);
+  G.fresh = false;
   k_reset_vertices<<<rgrid, 256, 0, st>>>(G.h, G.e, G.stamp, G.n, nullptr, 0);
   if (hipMemsetAsync(G.stats, 0, sizeof(unsigned long long) * kStatSlot * kStatSpread, st) != hipSuccess ||
       hipMemsetAsync(G.ctl, 0, sizeof(int) * CTL_SIZE, st) != hipSuccess ||
