@@ -254,8 +254,9 @@ mf_status mf_solve(mf_context *ctx, const mf_csr *g, mf_algo algo, mf_layout lay
 /* Autotuner (no reference counterpart): trial solves of the resident graph g with candidate tile
  * widths (class_lanes, VC only), relabel cadences (relabel_time_ratio) and one-workgroup BFS
  * limits (bfs_block_frontier), starting from *base (NULL = defaults), by coordinate descent over
- * the mean of three solves per candidate, sweeping all parameters again while a sweep changes
- * one (at most three sweeps); the winner is then timed against the starting options (six solves
+ * the mean of three solves per candidate (the current best's estimate is the mean of all its
+ * solves, re-timed before each sweep), with at least two sweeps over all parameters and a third
+ * if the second changed one; the winner is then timed against the starting options (six solves
  * each, alternating) and the faster set is kept.  *best receives it, *best_ms its mean solve time,
  * *trials the number of solves run (either may be NULL).  Every trial must produce the same flow
  * value, else MF_ERR_INTERNAL. */

@@ -1,7 +1,7 @@
 // Autotuner of the solver's launch parameters: trial solves over a small candidate set, by
 // coordinate descent (one parameter at a time, keeping the best so far).  The parameters
 // interact (tile widths, BFS hand-over and graph replay most), so the descent sweeps them all
-// again while a sweep still changes the options, at most kPasses times.
+// at least twice and again while a sweep still changes the options, at most kPasses times.
 //
 //   tile widths       mf_options.class_lanes: lanes per vertex of each VC degree class.  Narrow
 //                     tiles put more vertices in flight per wave, wide tiles more slots per
@@ -17,7 +17,9 @@
 //   batch algorithm   mf_options.batch_algo (batched solves only): TC or VC cycles.
 //
 // Lock-free schedules differ from run to run (solve times spread by +-15% on long-path graphs),
-// so a candidate's time is the mean of three solves, and the winner is checked at the end: it
+// so a candidate's time is the mean of three solves, the starting options' the mean of six, and
+// the current best's the mean of all its solves (it is timed again before every further sweep,
+// so a lucky low mean does not stop the search).  The winner is checked at the end: it
 // and the starting options are timed again over six solves each, alternating between the two so
 // that a drift of solve times during the run favours neither, and the faster of the two is
 // returned -- tuning never hands back options slower than those it started from.  Every trial
@@ -34,7 +36,7 @@ namespace {
 
 constexpr int kRepeats = 3;   // solves per candidate (mean)
 constexpr int kValidate = 6;  // solves per option set in the final check (alternating)
-constexpr int kPasses = 3;    // sweeps over all parameters, while a sweep changes something
+constexpr int kPasses = 3;    // sweeps over all parameters: two, then more while one changes something
 
 // VC tile-width presets (lanes for degree classes <=8, <=16, <=32, <=1024, larger)
 const uint16_t kLanePresets[][5] = {
@@ -124,8 +126,21 @@ mf_status descend(mf_options cur, bool tune_lanes, bool tune_batch_algo, Run run
   if (tune_lanes)
     for (int k = 0; k < 5; ++k)
       if (!cur.class_lanes[k]) cur.class_lanes[k] = kLanePresets[0][k];
-  double cur_ms = 0;
-  mf_status st = run(cur, cur_ms, kRepeats);
+  // cur_ms is the mean of every solve timed with cur so far: the options that won on a lucky
+  // low mean are timed again before each further sweep, so their estimate settles back and a
+  // faster candidate can still replace them
+  double cur_ms = 0, cur_sum = 0;
+  int cur_n = 0;
+  auto time_cur = [&](int reps) -> mf_status {
+    double ms = 0;
+    const mf_status s = run(cur, ms, reps);
+    if (s != MF_OK) return s;
+    cur_sum += ms * reps;
+    cur_n += reps;
+    cur_ms = cur_sum / cur_n;
+    return MF_OK;
+  };
+  mf_status st = time_cur(2 * kRepeats);
   if (st != MF_OK) return st;
   const mf_options start = cur;
   bool changed = false;
@@ -135,6 +150,8 @@ mf_status descend(mf_options cur, bool tune_lanes, bool tune_batch_algo, Run run
     if (s != MF_OK) return s;
     if (ms < cur_ms) {
       cur = t;
+      cur_sum = ms * kRepeats;
+      cur_n = kRepeats;
       cur_ms = ms;
       changed = true;
     }
@@ -142,9 +159,10 @@ mf_status descend(mf_options cur, bool tune_lanes, bool tune_batch_algo, Run run
   };
   for (int pass = 0; pass < kPasses; ++pass) {
     changed = false;
+    if (pass > 0 && (st = time_cur(kRepeats)) != MF_OK) return st;
     // sweep() reads cur live: each parameter is tried on top of the values chosen before it
     if ((st = sweep(cur, tune_lanes, tune_batch_algo, consider)) != MF_OK) return st;
-    if (!changed) break;
+    if (!changed && pass > 0) break;  // a second sweep always runs, against a re-timed best
   }
   if (std::memcmp(&cur, &start, sizeof(cur)) != 0) {
     double a_ms = 0, b_ms = 0;
