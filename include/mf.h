@@ -252,7 +252,9 @@ mf_status mf_solve(mf_context *ctx, const mf_csr *g, mf_algo algo, mf_layout lay
                    const mf_options *opts, mf_result *res, int32_t *flow_out);
 
 /* Autotuner (no reference counterpart): trial solves of the resident graph g with candidate tile
- * widths (class_lanes, VC only), relabel cadences (relabel_time_ratio) and one-workgroup BFS
+ * widths (class_lanes, VC only), relabel cadences (relabel_time_ratio, or a fixed
+ * cycles_per_relabel scaled from the cycles per round the starting options ran), forwarding
+ * (forward_hops), graph replay (cycle_graphs) and one-workgroup BFS
  * limits (bfs_block_frontier), starting from *base (NULL = defaults), by coordinate descent over
  * the mean of three solves per candidate (the current best's estimate is the mean of all its
  * solves, re-timed before each sweep), with at least two sweeps over all parameters and a third
