@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "../../include/mf.h"
+#include "solver.h"
 
 namespace {
 
@@ -274,6 +275,20 @@ mf_status mf_batch_autotune(mf_context *ctx, const mf_csr *inst, int32_t k, int3
   mf_options_default(&o);
   if (base) o = *base;
   o.profile_kernels = 0;
+  // the sample is packed and uploaded once; every trial solves it from the capacities again
+  std::vector<int32_t> off, dst, cap, src, snk, vbase;
+  mf_csr packed;
+  mf_status pst = mf::pack_instances(sub.data(), (int32_t)sub.size(), off, dst, cap, src, snk,
+                                     vbase, packed);
+  if (pst != MF_OK) return pst;
+  mf::BatchResident *resident = nullptr;
+  pst = mf::batch_upload(ctx, &packed, src.data(), snk.data(), vbase.data(), (int32_t)sub.size(),
+                         o, &resident);
+  if (pst != MF_OK) return pst;
+  struct Release {
+    mf::BatchResident *r;
+    ~Release() { mf::batch_release(r); }
+  } release{resident};
   std::vector<int64_t> ref, vals(sub.size());
   int32_t n = 0;
   auto run = [&](const mf_options &t, double &ms, int reps, double *cpr) -> mf_status {
@@ -281,8 +296,7 @@ mf_status mf_batch_autotune(mf_context *ctx, const mf_csr *inst, int32_t k, int3
     if (cpr) *cpr = 0;  // the batched kernel keeps its own cadence per instance
     for (int r = 0; r < reps; ++r) {
       mf_result res;
-      const mf_status st =
-          mf_batch_solve(ctx, sub.data(), (int32_t)sub.size(), &t, vals.data(), &res);
+      const mf_status st = mf::batch_solve_resident(ctx, resident, t, vals.data(), &res);
       if (st != MF_OK) return st;
       if (!ref.empty() && ref != vals) return MF_ERR_INTERNAL;
       ref = vals;

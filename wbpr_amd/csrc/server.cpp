@@ -16,13 +16,6 @@
 #include "mf_internal.h"
 #include "solver.h"
 
-namespace mf {
-mf_status pack_instances(const mf_csr *inst, int32_t k, std::vector<int32_t> &off,
-                         std::vector<int32_t> &dst, std::vector<int32_t> &cap,
-                         std::vector<int32_t> &src, std::vector<int32_t> &snk,
-                         std::vector<int32_t> &vbase, mf_csr &packed);
-}
-
 using namespace mf;
 
 namespace {

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <vector>
 
 #include "../../include/mf.h"
 
@@ -38,4 +39,21 @@ mf_status batch_solve_packed(mf_context *ctx, const mf_csr *packed, const int32_
                              const int32_t *snk, const int32_t *vbase, int32_t k, mf_algo algo,
                              mf_layout layout, const mf_options &o, int64_t *values,
                              mf_result *res);
+
+// Concatenates instances: vertex v of instance i becomes vbase[i] + v (batch.cpp).
+mf_status pack_instances(const mf_csr *inst, int32_t k, std::vector<int32_t> &off,
+                         std::vector<int32_t> &dst, std::vector<int32_t> &cap,
+                         std::vector<int32_t> &src, std::vector<int32_t> &snk,
+                         std::vector<int32_t> &vbase, mf_csr &packed);
+
+// A packed batch uploaded once and solved again and again, with other options each time (the
+// batched autotuner's trials): every solve restarts from the capacities on the device.
+// batch_mode is fixed by the upload; the algorithm comes from each solve's batch_algo.
+struct BatchResident;
+mf_status batch_upload(mf_context *ctx, const mf_csr *packed, const int32_t *src,
+                       const int32_t *snk, const int32_t *vbase, int32_t k,
+                       const mf_options &o, BatchResident **out);
+mf_status batch_solve_resident(mf_context *ctx, BatchResident *r, const mf_options &o,
+                               int64_t *values, mf_result *res);
+void batch_release(BatchResident *r);
 }  // namespace mf

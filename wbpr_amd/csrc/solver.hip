@@ -2744,6 +2744,59 @@ mf_status batch_solve_packed(mf_context *ctx, const mf_csr *packed, const int32_
   return st;
 }
 
+struct BatchResident {
+  DevGraph G;
+  std::vector<int32_t> vbase;
+  int32_t k = 0;
+  int32_t batch_mode = 0;
+};
+
+mf_status batch_upload(mf_context *ctx, const mf_csr *packed, const int32_t *src,
+                       const int32_t *snk, const int32_t *vbase, int32_t k,
+                       const mf_options &o, BatchResident **out) {
+  if (!out) return MF_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  BatchResident *r = new BatchResident;
+  r->vbase.assign(vbase, vbase + (size_t)k + 1);
+  r->k = k;
+  r->batch_mode = o.batch_mode;
+  const mf_status st = upload(ctx, packed, src, snk, k, MF_LAYOUT_BCSR, r->G);
+  if (st != MF_OK) {
+    batch_release(r);
+    return st;
+  }
+  *out = r;
+  return MF_OK;
+}
+
+mf_status batch_solve_resident(mf_context *ctx, BatchResident *r, const mf_options &o,
+                               int64_t *values, mf_result *res) {
+  if (!r) return MF_ERR_INVALID_ARGUMENT;
+  mf_status st = check_options(o);
+  if (st != MF_OK) return st;
+  if (o.batch_mode != r->batch_mode) return MF_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const mf_algo algo = o.batch_algo == 1 ? MF_ALGO_VC : MF_ALGO_TC;
+  st = r->batch_mode == 0 ? run_batch_wg(ctx, r->G, r->vbase.data(), algo, o, res)
+                          : solve_graph(ctx, r->G, algo, o, res);
+  if (res) res->upload_ms = 0;
+  if (st == MF_OK && values) {
+    std::vector<long long> v((size_t)r->k);
+    if (hipMemcpy(v.data(), r->G.values, sizeof(long long) * r->k, hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      return MF_ERR_DEVICE;
+    for (int32_t i = 0; i < r->k; ++i) values[i] = v[(size_t)i];
+  }
+  return st;
+}
+
+void batch_release(BatchResident *r) {
+  if (!r) return;
+  free_dev(&r->G);
+  delete r;
+}
+
 }  // namespace mf
 
 struct mf_graph {
