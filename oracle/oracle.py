@@ -8,6 +8,9 @@ checker; the product package (wbpr_amd) never imports it.
   certify          capacity / conservation / value / cut certificate of a flow
   brute_force_min_cut   minimum s-t cut by enumerating every vertex subset (n <= 18)
   ref_serial       runs the reference's own serial binary built by oracle/Makefile into _ref/
+  ref_layouts      runs the reference's own loaders and residual-layout builders (its
+                   maxflow-cuda / maxflow-bcsr / maxflow-bcsr2 src/graph.cpp, built with our
+                   dump driver ref_layouts.cpp into _ref/) and returns their arrays
 """
 from __future__ import annotations
 
@@ -21,6 +24,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 REF_BIN = os.path.join(HERE, "_ref", "maxflow-serial")
+REF_LAYOUTS = {k: os.path.join(HERE, "_ref", f"ref_layouts_{k}") for k in ("rcsr", "bcsr", "bcsr2")}
 
 if not os.path.exists(LIB):
     raise ImportError(f"{LIB} missing: run `make -C oracle`")
@@ -143,3 +147,25 @@ def ref_serial(path: str, file_type: int, s: int = 0, t: int = 1, timeout: float
     dt = time.perf_counter() - t0
     m = re.search(r"Max flow: (-?\d+)", out.stdout)
     return (int(m.group(1)) if m else None), dt
+
+
+def ref_layouts_available() -> bool:
+    return all(os.path.exists(p) for p in REF_LAYOUTS.values())
+
+
+def ref_layouts(layout: str, path: str, file_type: int, tmpdir: str) -> dict:
+    """Arrays the reference's host code builds from a graph file (ref_layouts.cpp): its
+    CSRGraph (csr_header = [n, m, m_processed, s, t], csr_offsets, csr_destinations,
+    csr_capacities) and its ResidualGraph for `layout` ("rcsr": offsets, destinations,
+    capacities, forward_flows, backward_flows, roffsets, rdestinations, flow_index; "bcsr": ...,
+    forward_flows, backward_idx; "bcsr2": ..., forward_flows of 2m entries)."""
+    out = os.path.join(tmpdir, f"ref_{layout}.txt")
+    subprocess.run([REF_LAYOUTS[layout], str(file_type), path, out], check=True,
+                   capture_output=True, timeout=600)
+    arrays = {}
+    with open(out) as f:
+        for line in f:
+            parts = line.split()
+            n = int(parts[1])
+            arrays[parts[0]] = np.array([int(x) for x in parts[2:2 + n]], dtype=np.int64)
+    return arrays
