@@ -2,34 +2,42 @@
 """Max-flow benchmark on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--algo tc|vc]
-                    [--layout bcsr|rcsr|bcsr2] [--no-cpu-baseline] [--no-variants]
+                    [--layout bcsr|rcsr|bcsr2] [--autotune | --default-options | --options JSON]
+                    [--no-graphs] [--no-batched] [--no-cpu-baseline] [--no-variants]
 
 Workloads
-  genrmf64 (default)  Genrmf a=b=64, 262,144 vertices: a step is one complete max-flow solve of the
-                      resident graph (each solve restarts from the capacities on the device)
-  rmat22, rlg64       the other single-instance configurations
-  batched             4096 bipartite matching networks (|L|, |R| log-uniform in [256, 65536],
-                      Poisson(8) arcs per left vertex) solved in one batched call; a step is one
-                      batched solve; compared with one-by-one single-instance solves
+  rmat22 (default)    R-MAT scale 22 (4,194,304 vertices, 65.2M arcs), VC on BCSR: a step is one
+                      complete max-flow solve of the resident graph (each solve restarts from the
+                      capacities on the device).  The same JSON line also carries the batched
+                      figures (4096 bipartite instances solved in one batched call against the
+                      same instances solved one by one) unless --no-batched
+  genrmf64, genrmf128, rlg64
+                      the other single-instance configurations
+  batched             the 4096-instance batched configuration alone (value = batched solves/s)
   serving             Poisson arrivals of mixed requests (RLG, Genrmf, bipartite; 1e3..1e6
                       vertices) through the continuous-batching server; p50/p99 latency
-  genrmf128           Genrmf a=b=128 (2,097,152 vertices), the size of the paper's Genrmf row
   copy                1 GiB device-to-device copies: the achievable HBM rate, and the known-size
                       kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters
+
+Launch options of the single-instance solves: by default the ones committed in TUNED below
+(found by this script's --autotune on an MI355X), so every run -- and the counter profiles
+under profiles/ -- use the same options; --autotune runs the autotuner on the resident graph
+first, --default-options uses the library defaults, --options sets them explicitly.
+--no-graphs turns captured cycle chunks off (the counter runs, whose profiler cannot replay
+graphs; the kernels and their arguments are the same).
 
 Every value is checked exactly: single instances against their committed values, batched and
 served results against solves of the same instances one by one.  Prints ONE JSON line (the
 driver's contract) with a `roofline` entry for the dominant kernel (algorithmic bytes per launch
-over the mean launch time, from HIP events around every launch on the solver's stream) and a
-`cpu_baseline` entry (the reference's own serial solver, or the CPU oracle where noted, on a
-bounded sample of the same workload).
+over the mean launch time, from HIP events on the solver's stream) and a `cpu_baseline` entry
+(the CPU oracle and the reference's own serial solver, on the same graph, one pinned core).
 """
 import argparse
 import ctypes
 import json
-import math
 import os
 import shutil
+import signal
 import subprocess
 import sys
 import tempfile
@@ -47,14 +55,14 @@ PROFILE_EVERY = 16  # cycle launches per launch timed with dispatch-packet event
 # single-instance workloads and their exact max-flow values (same numbers as
 # tests/golden/values.json, each computed by the CPU oracles)
 WORKLOADS = {
-    "genrmf64": dict(
-        desc="Genrmf a=64 b=64 c1=1 c2=10000 seed=1 (262,144 vertices, 1,290,240 arcs)",
-        gen=("gen_genrmf", (64, 64, 1, 10000, 1)), value=19996127, algo="vc",
-        variants=("tc", "vc")),
     "rmat22": dict(
         desc="R-MAT scale 22 edge factor 16 a=.57 b=.19 c=.19 caps [1,1000] seed=1 "
              "(4,194,304 vertices, 65,244,556 arcs after dropping self-loops and duplicates)",
         gen=("gen_rmat", (22, 16, 0.57, 0.19, 0.19, 1000, 1)), value=18886759, algo="vc"),
+    "genrmf64": dict(
+        desc="Genrmf a=64 b=64 c1=1 c2=10000 seed=1 (262,144 vertices, 1,290,240 arcs)",
+        gen=("gen_genrmf", (64, 64, 1, 10000, 1)), value=19996127, algo="vc",
+        variants=("tc", "vc")),
     "genrmf128": dict(
         desc="Genrmf a=128 b=128 c1=1 c2=10000 seed=1 (2,097,152 vertices, 10,403,840 arcs)",
         gen=("gen_genrmf", (128, 128, 1, 10000, 1)), value=81048332, algo="vc",
@@ -63,49 +71,103 @@ WORKLOADS = {
         desc="Washington RLG 64x64 r=10000 seed=1 (4,098 vertices, 12,224 arcs)",
         gen=("gen_washington_rlg", (64, 64, 10000, 1)), value=466088, algo="tc"),
 }
-CPU_SAMPLE = dict(desc="Genrmf a=14 b=14 c1=1 c2=10000 seed=1 (2,744 vertices, 12,740 arcs)",
-                  gen=("gen_genrmf", (14, 14, 1, 10000, 1)), value=919190)
+
+# Launch options per workload and algorithm, found by
+#   python bench.py --workload <w> --algo <a> --autotune --no-batched --no-variants --no-cpu-baseline
+# on one MI355X (the autotuner's coordinate descent over trial solves of the resident graph).
+# Workloads not listed run the library defaults.
+TUNED = {}
+
+BATCH_K = 4096
+BATCH_SEED = 2024
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def splitmix(x):
-    x = (x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
-    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
-    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
-    return x ^ (x >> 31)
-
-
-def cpu_baseline_reference(mf):
-    """Reference serial solver (built from its sources) on the bounded sample, one core."""
-    ref = os.path.join(ROOT, "oracle", "_ref", "maxflow-serial")
-    if not os.path.exists(ref):
-        return None
-    g = getattr(mf, CPU_SAMPLE["gen"][0])(*CPU_SAMPLE["gen"][1])
-    tmp = tempfile.mkdtemp()
+def host_info():
+    """Host CPU model and the cores this process may run on (the GPU box's share)."""
+    model = "unknown"
     try:
-        path = os.path.join(tmp, "sample.max")
-        g.save_dimacs(path)
-        cmd = [ref, "-v", "2", "-f", path, "-s", "0", "-t", "1"]
-        if shutil.which("taskset"):
-            cmd = ["taskset", "-c", "0"] + cmd
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = list(range(os.cpu_count() or 1))
+    return {"cpu_model": model, "nproc": len(cores), "os_cpu_count": os.cpu_count()}, cores
+
+
+class pinned:
+    """Pins the calling thread to one core for the duration of a CPU-baseline leg."""
+
+    def __init__(self, core):
+        self.core = core
+
+    def __enter__(self):
+        self.old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {self.core})
+
+    def __exit__(self, *a):
+        os.sched_setaffinity(0, self.old)
+
+
+def cpu_dinic_same_graph(g, expect, core):
+    """The CPU oracle (Dinic, C, one thread pinned to one core) on the benchmarked graph."""
+    from oracle import oracle as orc
+
+    with pinned(core):
         t0 = time.perf_counter()
-        out = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        v = orc.dinic_maxflow(g)
         dt = time.perf_counter() - t0
+    if v != expect:
+        raise SystemExit(f"cpu baseline: Dinic value {v} != exact {expect}")
+    return dt
+
+
+def cpu_reference_serial(g, core, cap_s):
+    """The reference maxflow-serial (built from its sources into oracle/_ref) on the same graph,
+    written as a DIMACS file, under a wall-clock cap.  Returns a dict; `seconds` is None when it
+    did not finish (reported as "> cap")."""
+    from oracle import oracle as orc
+
+    if not orc.ref_available():
+        return {"available": False}
+    base = "/dev/shm" if os.path.isdir("/dev/shm") else None
+    tmp = tempfile.mkdtemp(dir=base)
+    path = os.path.join(tmp, "graph.max")
+    try:
+        t0 = time.perf_counter()
+        g.save_dimacs(path)
+        write_s = time.perf_counter() - t0
+        cmd = [orc.REF_BIN, "-v", "2", "-f", path, "-s", "0", "-t", "1"]
+        if shutil.which("taskset"):
+            cmd = ["taskset", "-c", str(core)] + cmd
+        t0 = time.perf_counter()
+        proc = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                                start_new_session=True)
+        try:
+            out, _ = proc.communicate(timeout=cap_s)
+            dt = time.perf_counter() - t0
+            got = None
+            for line in out.splitlines():
+                if line.startswith("Max flow:"):
+                    got = int(line.split()[-1])
+            return {"available": True, "cap_s": cap_s, "seconds": round(dt, 2), "value": got,
+                    "dimacs_write_s": round(write_s, 2)}
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.communicate()
+            return {"available": True, "cap_s": cap_s, "seconds": None,
+                    "dimacs_write_s": round(write_s, 2)}
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
-    got = None
-    for line in out.stdout.splitlines():
-        if line.startswith("Max flow:"):
-            got = int(line.split()[-1])
-    if got != CPU_SAMPLE["value"]:
-        raise SystemExit(f"cpu baseline value {got} != {CPU_SAMPLE['value']}")
-    return {"value": round(2 * g.m / dt, 1), "unit": "residual edges/s", "cores": 1,
-            "kind": "reference",
-            "sample": f"maxflow-serial (reference sources) on {CPU_SAMPLE['desc']}: "
-                      f"{dt:.2f} s wall, pinned to core 0; value = residual slots / time"}
 
 
 def kernel_traffic(kernel, workload):
@@ -122,7 +184,7 @@ def kernel_traffic(kernel, workload):
 
 
 def roofline_of(results, workload):
-    """Roofline of the dominant kernel over solves run with profile_kernels=1: algorithmic bytes
+    """Roofline of the dominant kernel over solves run with profile_kernels: algorithmic bytes
     per launch over the mean launch time, both summed over every profiled solve."""
     kernel_ms = sum(r.kernel_ms for r in results)
     relabel_ms = sum(r.relabel_ms for r in results)
@@ -153,53 +215,78 @@ def line_common(args, value, unit, ms_per_step, config, roofline, cpu, higher=Tr
             "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu}
 
 
+def device_label(ctx):
+    name, ncu = ctx.device_info()
+    label = name.strip()
+    if label.startswith("("):  # the runtime reported no marketing name
+        label = label.strip("()").split(":")[0]
+    return f"{label}, {ncu} CUs", ncu
+
+
+def solve_options(args, workload, algo_name):
+    """Options of the timed solves (see the module docstring) and where they came from."""
+    if args.options:
+        opts, src = json.loads(args.options), "--options"
+    elif args.default_options:
+        opts, src = {}, "library defaults"
+    else:
+        opts = dict(TUNED.get((workload, algo_name), {}))
+        src = "committed tuned options" if opts else "library defaults (none committed)"
+    if args.cycles:
+        opts["cycles_per_relabel"] = args.cycles
+    if args.no_graphs:
+        opts["cycle_graphs"] = 0
+    return opts, src
+
+
 # ---------------------------------------------------------------------------------------------
 def bench_single(args, mf):
     W = WORKLOADS[args.workload]
+    t0 = time.perf_counter()
     g = getattr(mf, W["gen"][0])(*W["gen"][1])
+    gen_s = time.perf_counter() - t0
     algo_name = args.algo or W["algo"]
     algo, layout = mf.ALGOS[algo_name], mf.LAYOUTS[args.layout]
-    opts = dict(cycles_per_relabel=args.cycles) if args.cycles else {}
     ctx = mf.Context(0)
-    dev_name, ncu = ctx.device_info()
-    log(f"device {dev_name}, {ncu} CUs; workload {W['desc']}")
+    dev, _ = device_label(ctx)
+    log(f"device {dev}; workload {W['desc']} (generated in {gen_s:.1f} s)")
     t_up = time.perf_counter()
-    dg = ctx.upload(g, layout)  # host layout build + host-to-device copy
+    dg = ctx.upload(g, layout)  # device layout build from the host CSR + host-to-device copy
     upload_ms = 1e3 * (time.perf_counter() - t_up)
 
     def solve_checked(**kw):
-        r = dg.solve(algo, **{**opts, **kw})
+        r = dg.solve(algo, **kw)
         if r.value != W["value"]:
             raise SystemExit(f"max-flow value {r.value} != exact {W['value']}")
         return r
 
-    for _ in range(args.warmup):
-        solve_checked()
-    ctx.synchronize()
-    # launch parameters from the autotuner (trial solves of this graph), before the timed solves
-    tuned, trials = {}, 0
-    if not args.no_autotune:
+    opts, opts_src = solve_options(args, args.workload, algo_name)
+    trials = 0
+    if args.autotune:
         tuned, tune_ms, trials = dg.autotune(algo, **opts)
-        tuned = {k: v for k, v in tuned.items() if k not in ("batch_algo", "batch_mode")}
-        log(f"autotune: {trials} trial solves, best {tune_ms:.2f} ms with {tuned}")
+        opts = {k: v for k, v in tuned.items() if k not in ("batch_algo", "batch_mode")}
+        if args.no_graphs:
+            opts["cycle_graphs"] = 0
+        opts_src = f"autotuned in this run ({trials} trial solves)"
+        log(f"autotune: {trials} trial solves, best {tune_ms:.2f} ms with {json.dumps(opts)}")
+    for _ in range(args.warmup):
+        solve_checked(**opts)
+    ctx.synchronize()
     # every 16th cycle launch of the timed solves carries start/end events in its dispatch
     # packet (a timed launch costs a cache flush, so only a sample is timed)
     t0 = time.perf_counter()
-    results = [solve_checked(profile_kernels=PROFILE_EVERY, **tuned) for _ in range(args.steps)]
+    results = [solve_checked(profile_kernels=PROFILE_EVERY, **opts) for _ in range(args.steps)]
     ctx.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / args.steps
     slots = 2 * g.m
     roofline = roofline_of(results, args.workload)
+    solve_ms = [r.solve_ms for r in results]
     t1 = time.perf_counter()
-    plain = [solve_checked(**tuned) for _ in range(min(3, args.steps))]
+    plain = [solve_checked(**opts) for _ in range(min(3, args.steps))]
     plain_ms = 1e3 * (time.perf_counter() - t1) / len(plain)
-    t1 = time.perf_counter()
-    dflt = [solve_checked() for _ in range(min(3, args.steps))]
-    default_ms = 1e3 * (time.perf_counter() - t1) / len(dflt)
-    log(f"{algo_name}/{args.layout}: {ms_per_step:.2f} ms per solve with launch events, "
-        f"{plain_ms:.2f} ms without, {default_ms:.2f} ms with default options")
-
+    log(f"{algo_name}/{args.layout}: {ms_per_step:.2f} ms per solve with launch events "
+        f"(min {min(solve_ms):.2f}, max {max(solve_ms):.2f}), {plain_ms:.2f} ms without")
     variants = {}
     if not args.no_variants:
         # thread-centric scanning leaves one lane per hub vertex on power-law graphs (minutes
@@ -209,49 +296,67 @@ def bench_single(args, mf):
             for vl in ("rcsr", "bcsr", "bcsr2"):
                 vg = ctx.upload(g, mf.LAYOUTS[vl])
                 times = []
-                for _ in range(2):
-                    r = vg.solve(mf.ALGOS[va], **opts)
+                for _ in range(3):
+                    r = vg.solve(mf.ALGOS[va])
                     if r.value != W["value"]:
                         raise SystemExit(f"{va}/{vl}: value {r.value} != exact {W['value']}")
                     times.append(r.solve_ms)
                 vg.close()
                 variants[f"{va}/{vl}"] = round(min(times), 3)
-                log(f"variant {va}/{vl}: best of 2 solves {variants[f'{va}/{vl}']} ms")
-    cpu = None if args.no_cpu_baseline else cpu_baseline_reference(mf)
+                log(f"variant {va}/{vl} (default options): best of 3 solves "
+                    f"{variants[f'{va}/{vl}']} ms")
+    dg.close()
     last = results[-1]
     config = {"workload": W["desc"], "algo": algo_name, "layout": args.layout,
-              "max_flow": W["value"], "residual_slots": slots, "global_relabels": last.rounds,
-              "cycles": last.cycles, "pushes": last.pushes, "relabels": last.relabels,
-              "slots_scanned": last.slots_scanned, "upload_ms": round(upload_ms, 2),
-              "autotuned_options": tuned, "autotune_trials": trials,
-              "untimed_ms_per_solve": round(plain_ms, 3),
-              "default_options_ms_per_solve": round(default_ms, 3),
-              "host_inclusive_residual_edges_per_s": round(slots / ((upload_ms + ms_per_step) * 1e-3), 1),
+              "max_flow": W["value"], "value_is": "residual slots of the graph (2m) per second "
+              "of solve wall time, graph resident in HBM",
+              "residual_slots": slots, "solve_ms_mean": round(ms_per_step, 3),
+              "solve_ms_min": round(min(solve_ms), 3), "solve_ms_max": round(max(solve_ms), 3),
+              "global_relabels": last.rounds, "cycles": last.cycles, "pushes": last.pushes,
+              "relabels": last.relabels, "slots_scanned": last.slots_scanned,
               "scanned_slots_per_s": round(last.slots_scanned / (last.solve_ms * 1e-3), 1),
-              "device": dev_name, "variants_solve_ms": variants}
-    return line_common(args, round(slots / (wall / args.steps), 1), "residual edges/s",
-                       ms_per_step, config, roofline, cpu)
+              "kernel_ms": round(last.kernel_ms, 3), "relabel_ms": round(last.relabel_ms, 3),
+              "upload_ms": round(upload_ms, 2),
+              "host_inclusive_residual_edges_per_s":
+                  round(slots / ((upload_ms + ms_per_step) * 1e-3), 1),
+              "options": opts, "options_source": opts_src, "autotune_trials": trials,
+              "untimed_ms_per_solve": round(plain_ms, 3), "device": dev,
+              "variants_solve_ms": variants}
+    value = round(slots / (wall / args.steps), 1)
+    cpu = None
+    if not args.no_cpu_baseline:
+        info, cores = host_info()
+        core = cores[0]
+        log(f"cpu baseline: Dinic on the same graph, pinned to core {core} ({info['cpu_model']})")
+        dinic_s = cpu_dinic_same_graph(g, W["value"], core)
+        log(f"cpu baseline: Dinic {dinic_s:.2f} s; reference serial under a {args.ref_cap} s cap")
+        ref = cpu_reference_serial(g, core, args.ref_cap)
+        if not ref.get("available"):
+            ref_txt = "the reference maxflow-serial binary is not built"
+        elif ref["seconds"] is None:
+            ref_txt = (f"the reference maxflow-serial (built from its sources) on the same graph "
+                       f"as DIMACS did not finish within its {args.ref_cap} s cap (> {args.ref_cap} s)")
+        else:
+            ref_txt = (f"the reference maxflow-serial on the same graph: {ref['seconds']} s "
+                       f"(value {ref['value']})")
+        cpu = {"value": round(slots / dinic_s, 1), "unit": "residual edges/s", "cores": 1,
+               "kind": "port",
+               "sample": f"the whole benchmarked graph ({W['desc']}): CPU oracle (Dinic, C, one "
+                         f"thread pinned to core {core}) {dinic_s:.2f} s per solve, value = 2m / "
+                         f"solve time; {ref_txt}",
+               "seconds": round(dinic_s, 3), "host": info, "reference_serial": ref}
+        log(f"cpu baseline: {ref_txt}")
+    return line_common(args, value, "residual edges/s", ms_per_step, config, roofline, cpu)
 
 
 # ---------------------------------------------------------------------------------------------
-def batched_mix(mf, k, seed, lo=256, hi=65536):
-    rng = np.random.default_rng(seed)
-    graphs = []
-    for i in range(k):
-        L = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
-        R = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
-        graphs.append(mf.gen_bipartite(L, R, 8.0, seed=splitmix(seed * 1000003 + i)))
-    return graphs
-
-
-def cpu_baseline_batched(graphs, values, sample):
-    """CPU oracle (Dinic), one instance per thread over all host cores, on a sample."""
+def cpu_baseline_batched(graphs, values, sample, cores):
+    """CPU oracle (Dinic), one instance per thread over the host cores, on a sample."""
     from concurrent.futures import ThreadPoolExecutor
 
     from oracle import oracle as orc
 
     idx = list(range(0, len(graphs), max(1, len(graphs) // sample)))[:sample]
-    cores = min(16, os.cpu_count() or 1)
     t0 = time.perf_counter()
     with ThreadPoolExecutor(cores) as ex:
         got = list(ex.map(lambda i: orc.dinic_maxflow(graphs[i]), idx))
@@ -265,7 +370,11 @@ def cpu_baseline_batched(graphs, values, sample):
                       f"{cores} threads, {dt:.2f} s"}
 
 
-def bench_batched(args, mf):
+def run_batched(args, mf):
+    """The 4096-instance batched configuration: one batched call against the same instances
+    solved one by one with the single-instance kernels.  Returns (summary dict, mean ms)."""
+    from wbpr_amd.workloads import batched_mix
+
     k = args.instances
     t0 = time.perf_counter()
     graphs = batched_mix(mf, k, args.seed)
@@ -274,19 +383,19 @@ def bench_batched(args, mf):
     log(f"generated {k} bipartite instances: {nv} vertices, {na} arcs "
         f"({time.perf_counter() - t0:.1f} s)")
     ctx = mf.Context(0)
-    dev_name, ncu = ctx.device_info()
+    dev, _ = device_label(ctx)
     algo_mode = 1 if args.algo == "vc" else 0  # batch_algo: 0 TC, 1 VC
     tuned, trials = {"batch_algo": algo_mode}, 0
-    if not args.no_autotune:
+    if not args.no_batch_autotune:
         t1 = time.perf_counter()
         tuned, tune_ms, trials = ctx.batch_autotune(graphs, sample=args.tune_sample,
                                                     batch_algo=algo_mode)
         log(f"batch autotune: {trials} trial batches of {args.tune_sample} instances in "
             f"{time.perf_counter() - t1:.1f} s, best {tune_ms:.2f} ms with {tuned}")
         algo_mode = tuned["batch_algo"]
-    runs = []
+    runs, calls = [], []
     values = None
-    for step in range(args.warmup + args.steps):
+    for step in range(args.batch_warmup + args.batch_steps):
         t1 = time.perf_counter()
         vals, res = ctx.batch_solve(graphs, **tuned)
         wall = time.perf_counter() - t1
@@ -295,11 +404,12 @@ def bench_batched(args, mf):
         values = vals
         log(f"batched solve: device {res.solve_ms:.1f} ms, call {1e3 * wall:.1f} ms, "
             f"rounds {res.rounds}, cycles {res.cycles}")
-        if step >= args.warmup:
+        if step >= args.batch_warmup:
             runs.append(res)
+            calls.append(1e3 * wall)
     solve_ms = float(np.mean([r.solve_ms for r in runs]))
     batched_sps = k / (solve_ms * 1e-3)
-    # one by one on a sample of the same mix, graphs resident, same kernels
+    # one by one on a random sample of the same mix, each graph resident, single-instance solver
     rng = np.random.default_rng(args.seed + 1)
     sample = sorted(rng.choice(k, size=min(args.one_by_one, k), replace=False).tolist())
     per = []
@@ -313,16 +423,31 @@ def bench_batched(args, mf):
     one_sps = 1e3 / float(np.mean(per))
     log(f"batched {batched_sps:.1f} solves/s, one-by-one {one_sps:.1f} solves/s "
         f"(mean {np.mean(per):.2f} ms over {len(per)} instances): x{batched_sps / one_sps:.2f}")
-    cpu = None if args.no_cpu_baseline else cpu_baseline_batched(graphs, values, args.cpu_sample)
-    config = {"workload": f"{k} bipartite matching networks, |L|,|R| log-uniform [256,65536], "
-                          f"Poisson(8) arcs per left vertex, unit capacities, seed {args.seed}",
-              "vertices": nv, "arcs": na, "algo": "vc" if algo_mode else "tc", "layout": "bcsr",
-              "batched_solves_per_s": round(batched_sps, 2),
-              "one_by_one_solves_per_s": round(one_sps, 2),
-              "one_by_one_sample": len(per), "speedup_vs_one_by_one": round(batched_sps / one_sps, 2),
-              "autotuned_options": tuned, "autotune_trials": trials,
-              "autotune_sample": args.tune_sample, "device": dev_name}
-    return line_common(args, round(batched_sps, 2), "solves/s", solve_ms, config,
+    cpu = None
+    if not args.no_cpu_baseline:
+        info, cores = host_info()
+        cpu = cpu_baseline_batched(graphs, values, args.cpu_sample, min(16, len(cores)))
+        cpu["host"] = info
+    summary = {"workload": f"{k} bipartite matching networks, |L|,|R| log-uniform [256,65536], "
+                           f"Poisson(8) arcs per left vertex, unit capacities, seed {args.seed}",
+               "vertices": nv, "arcs": na, "algo": "vc" if algo_mode else "tc",
+               "layout": "bcsr", "batched_solves_per_s": round(batched_sps, 2),
+               "batched_device_ms": round(solve_ms, 3),
+               "batched_call_ms": round(float(np.mean(calls)), 1),
+               "one_by_one_solves_per_s": round(one_sps, 2),
+               "one_by_one_sample": len(per),
+               "speedup_vs_one_by_one": round(batched_sps / one_sps, 2),
+               "options": tuned, "autotune_trials": trials,
+               "autotune_sample": args.tune_sample if trials else 0, "device": dev,
+               "cpu_baseline": cpu}
+    return summary, solve_ms
+
+
+def bench_batched(args, mf):
+    summary, solve_ms = run_batched(args, mf)
+    cpu = summary.pop("cpu_baseline")
+    args.steps, args.warmup = args.batch_steps, args.batch_warmup
+    return line_common(args, summary["batched_solves_per_s"], "solves/s", solve_ms, summary,
                        None, cpu)
 
 
@@ -363,33 +488,13 @@ def bench_copy(args):
                         "traffic": 2 * nbytes}, None)
 
 
-def serving_requests(mf, n, seed):
-    """Mixed requests, 1e3..1e6 vertices log-uniform, generator chosen uniformly."""
-    rng = np.random.default_rng(seed)
-    reqs = []
-    for i in range(n):
-        nv = math.exp(rng.uniform(math.log(1e3), math.log(1e6)))
-        kind = int(rng.integers(0, 3))
-        s = splitmix(seed * 7919 + i)
-        if kind == 0:
-            side = max(3, int(round(math.sqrt(nv))))
-            g = mf.gen_washington_rlg(side, side, 10000, s)
-        elif kind == 1:
-            a = max(2, int(round(nv ** (1 / 3))))
-            g = mf.gen_genrmf(a, a, 1, 10000, s)
-        else:
-            half = max(2, int(round(nv / 2)))
-            g = mf.gen_bipartite(half, half, 8.0, s)
-        reqs.append(g)
-    return reqs
-
-
 def bench_serving(args, mf):
-    reqs = serving_requests(mf, args.requests, args.seed)
-    rng = np.random.default_rng(args.seed + 5)
-    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=len(reqs)))
+    from wbpr_amd.workloads import poisson_arrivals, serving_requests
+
+    reqs, kinds = serving_requests(mf, args.requests, args.seed)
+    arrivals = poisson_arrivals(len(reqs), args.rate, args.seed + 5)
     ctx = mf.Context(0)
-    dev_name, _ = ctx.device_info()
+    dev, _ = device_label(ctx)
     srv = mf.Server(ctx, hbm_budget_bytes=args.budget_gb << 30, max_inflight_launches=args.inflight)
     done_at = {}
     ids = {}
@@ -430,7 +535,7 @@ def bench_serving(args, mf):
               "solves_per_s": round(sps, 3), "offered_load_per_s": args.rate,
               "inflight_launches": args.inflight, "launches": stats["launches"],
               "arena_peak_bytes": stats["arena_peak_bytes"], "checked_alone": len(check),
-              "device": dev_name}
+              "device": dev}
     return line_common(args, round(sps, 3), "solves/s", 1e3 * total / len(reqs), config,
                        None, None)
 
@@ -440,20 +545,33 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="genrmf64",
+    ap.add_argument("--workload", default="rmat22",
                     choices=sorted(WORKLOADS) + ["batched", "serving", "copy"])
     ap.add_argument("--algo", default=None, choices=["tc", "vc"])
     ap.add_argument("--layout", default="bcsr", choices=["bcsr", "rcsr", "bcsr2"])
     ap.add_argument("--cycles", type=int, default=0, help="push-relabel cycles per global relabel")
+    grp = ap.add_mutually_exclusive_group()
+    grp.add_argument("--autotune", action="store_true",
+                     help="run the autotuner on the resident graph before the timed solves")
+    grp.add_argument("--default-options", action="store_true", help="library default options")
+    grp.add_argument("--options", default=None, help="explicit options as a JSON object")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="no captured cycle chunks (counter runs)")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="single-instance workloads: leave out the batched figures")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true", help="default launch parameters")
+    ap.add_argument("--ref-cap", type=float, default=30.0,
+                    help="wall-clock cap (s) of the reference serial solver's run")
+    ap.add_argument("--no-batch-autotune", action="store_true")
     ap.add_argument("--tune-sample", type=int, default=1024,
                     help="batched: autotune sample (enough instances to fill every workgroup)")
-    ap.add_argument("--instances", type=int, default=4096, help="batched: number of instances")
-    ap.add_argument("--one-by-one", type=int, default=256, help="batched: one-by-one sample size")
+    ap.add_argument("--instances", type=int, default=BATCH_K, help="batched: number of instances")
+    ap.add_argument("--batch-steps", type=int, default=2, help="batched: timed batched calls")
+    ap.add_argument("--batch-warmup", type=int, default=1, help="batched: untimed batched calls")
+    ap.add_argument("--one-by-one", type=int, default=512, help="batched: one-by-one sample size")
     ap.add_argument("--cpu-sample", type=int, default=64, help="batched: CPU baseline sample")
-    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--seed", type=int, default=BATCH_SEED)
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=8.0, help="serving: arrivals per second")
     ap.add_argument("--inflight", type=int, default=4, help="serving: concurrent launches")
@@ -469,13 +587,14 @@ def main():
     import wbpr_amd as mf
 
     if args.workload == "batched":
-        if args.steps == 10 and args.warmup == 2:  # defaults sized for the single solve
-            args.steps, args.warmup = 1, 0
         line = bench_batched(args, mf)
     elif args.workload == "serving":
         line = bench_serving(args, mf)
     else:
         line = bench_single(args, mf)
+        if not args.no_batched:
+            summary, _ = run_batched(args, mf)
+            line["config"]["batched"] = summary
     print(json.dumps(line), flush=True)
 
 

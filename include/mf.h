@@ -268,8 +268,15 @@ mf_status mf_autotune(mf_context *ctx, mf_graph *g, mf_algo algo, const mf_optio
 /* Drop-in for the reference host->solver interface on RCSR arrays:
  * preflow() (maxflow-cuda/src/preflow.cu:3-31) followed by push_relabel()
  * (maxflow-cuda/src/push_relabel.cu:35-221), declared at maxflow-cuda/include/parallel_graph.cuh:44-54.
- * Same host arrays and meaning: on return h, e, fflows, bflows hold the final state, *excess_total
- * the total excess that reached the sink, and e[t] the max-flow value.  flow_idx is used as given. */
+ * Same host arrays and meaning: on return h, e, fflows, bflows hold the final state and e[t] the
+ * max-flow value.  flow_idx is used as given.  Two differences from the reference:
+ *   - the solve always starts from the capacities: the caller's cpu_fflows / cpu_bflows, heights
+ *     and excesses on entry are ignored (preflow() resets them in the reference too, except that
+ *     it keeps the flows of arcs out of other vertices; here every arc starts at residual = cap);
+ *   - *excess_total receives e[t], the flow value.  The reference's Excess_total is the preflow's
+ *     total minus the excess its global relabel found stranded (push_relabel.cu:114-119,
+ *     global_relabel.cu:143-177), which equals e[t] only once every stranded unit is counted;
+ *     here stranded excess stays where it is (a maximum preflow) and the value is reported. */
 mf_status mf_rcsr_push_relabel(mf_context *ctx, int algo_type, int V, int E, int source, int sink,
                                int *cpu_height, int *cpu_excess_flow,
                                const int *cpu_offsets, const int *cpu_destinations,
@@ -294,28 +301,59 @@ mf_status mf_batch_autotune(mf_context *ctx, const mf_csr *inst, int32_t k, int3
                             int32_t *trials);
 
 /* ---- serving (no reference counterpart) ----
- * A request loop over a device-resident arena of hbm_budget_bytes.  Submitted graphs are copied
- * (the caller's arrays may be freed after submit) and admitted into the arena when their
- * footprint fits; admitted requests are solved by batched launches that overlap on the device;
- * finished requests are retired and their memory reused. */
+ * Continuous batching.  The server allocates its device arena once, from half of
+ * hbm_budget_bytes, and keeps a persistent serving kernel running while requests arrive.
+ * Submitted graphs are copied (the caller's arrays may be freed after submit).  A request of at
+ * most 16,384 vertices and 262,144 arcs is laid out on the host at submit, admitted as soon as
+ * arena ranges and a queue entry are free, copied into its ranges and published into the
+ * running kernel's queue, where the next free workgroup solves it inside the running batch; its
+ * ranges are reused when it finishes.  A larger request runs alone on the whole grid from one of
+ * max_inflight_launches worker contexts (each with its own stream), with device memory charged
+ * to the other half of the budget (mf_footprint_bytes); those solves overlap on the device with
+ * the serving kernel.  Admission goes through pending requests in arrival order and admits each
+ * one that fits now.  Every call takes the server's lock, so calls may come from several
+ * threads. */
 typedef struct mf_server mf_server;
 typedef struct mf_server_stats {
   int64_t submitted, admitted, completed, rejected;
-  int64_t launches;
-  int64_t arena_bytes, arena_peak_bytes;
+  int64_t launches;               /* serving-kernel launches + requests solved alone */
+  int64_t arena_bytes, arena_peak_bytes;  /* budget in use now / at most: arena ranges of
+                                             batched requests + device memory of the others */
+  int64_t batched, solo;          /* completed requests, by path */
+  int64_t serve_kernel_launches;  /* launches of the persistent serving kernel */
 } mf_server_stats;
+/* Host clock of a completed request, in ms since the server was created. */
+typedef struct mf_server_times {
+  double submit_ms;   /* mf_server_submit returned */
+  double admit_ms;    /* admitted: arena ranges allocated and its copy queued, or its own launch */
+  double start_ms;    /* published into the running batch, or its launch started */
+  double finish_ms;   /* the server saw its result */
+  double device_ms;   /* the request's own solve on the device */
+  int32_t batched;    /* 1: solved inside the running batch; 0: alone on the whole grid */
+  int32_t reserved;
+} mf_server_times;
 mf_status mf_server_create(mf_context *ctx, int64_t hbm_budget_bytes, int32_t max_inflight_launches,
                            mf_server **out);
 /* Submit a request; *request_id receives its id.  MF_ERR_BUDGET if it can never fit. */
 mf_status mf_server_submit(mf_server *srv, const mf_csr *g, int64_t *request_id);
-/* Advance the loop: admit what fits, launch, retire finished launches.  Non-blocking unless
- * wait != 0 (then blocks until at least one request completes or nothing is pending). */
+/* Advance the loop: collect finished requests, admit what fits, publish finished copies into
+ * the running batch, launch the serving kernel again if no workgroup of it is left.
+ * Non-blocking unless wait != 0 (then returns once at least one request completes or nothing is
+ * pending or in flight). */
 mf_status mf_server_progress(mf_server *srv, int32_t wait);
 /* Result of a request: MF_ERR_NOT_READY until it has completed. */
 mf_status mf_server_result(mf_server *srv, int64_t request_id, int64_t *value, double *device_ms);
+/* Times of a completed request (MF_ERR_NOT_READY until it has completed). */
+mf_status mf_server_request_times(mf_server *srv, int64_t request_id, mf_server_times *out);
+/* Latency (finish - submit) percentiles over the completed requests (linear interpolation
+ * between ranks), and completed requests per second from the first submit to the last finish.
+ * MF_ERR_NOT_READY while none has completed. */
+mf_status mf_server_latency(mf_server *srv, double *p50_ms, double *p99_ms, double *solves_per_s);
 mf_status mf_server_stats_get(mf_server *srv, mf_server_stats *st);
+/* Finishes the requests already on the GPU, drops pending ones, frees everything. */
 mf_status mf_server_destroy(mf_server *srv);
-/* Bytes of device memory a graph occupies once admitted (the admission footprint). */
+/* Device bytes of a graph solved alone: the larger of the upload's peak (layout, raw CSR copy
+ * and device-build buffers) and what stays resident (the admission footprint of large requests). */
 int64_t mf_footprint_bytes(int32_t n, int64_t m);
 
 #ifdef __cplusplus

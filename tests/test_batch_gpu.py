@@ -108,9 +108,13 @@ def test_server_continuous_batching(mf, ctx, golden):
 
 
 def test_server_budget(mf, ctx, golden):
+    # a 400 KB budget: a 200 KB arena (1,307 vertices, 13,070 slots) takes Genrmf 8x8 (513
+    # vertices, 4,480 slots); Genrmf 16x16 fits neither the arena nor, alone (1.4 MB), the other
+    # half of the budget, so it can never run
     g, v = generated_graph(mf, golden, "genrmf16_s1")
-    small = mf.footprint_bytes(g.n, g.m) - 1
-    srv = mf.Server(ctx, hbm_budget_bytes=small, max_inflight_launches=1)
+    budget = 400 << 10
+    assert mf.footprint_bytes(g.n, g.m) > budget // 2
+    srv = mf.Server(ctx, hbm_budget_bytes=budget, max_inflight_launches=1)
     try:
         with pytest.raises(mf.MFError) as ei:
             srv.submit(g)
@@ -165,5 +169,43 @@ def test_server_large_requests_run_alone(mf, orc, ctx, golden):
         assert not pending
         st = srv.stats()
         assert st["completed"] == len(ids) and st["launches"] >= 3
+    finally:
+        srv.close()
+
+
+def test_server_joins_running_batch(mf, orc, ctx):
+    # requests submitted while earlier ones are still on the GPU are published into the same
+    # running serving kernel: a second group starts before the first group has finished, and
+    # one launch serves both
+    g1 = [mf.gen_bipartite(8000, 8000, 8.0, seed=300 + i) for i in range(4)]
+    g2 = [mf.gen_bipartite(8000, 8000, 8.0, seed=400 + i) for i in range(4)]
+    expect = {id(g): orc.dinic_maxflow(g) for g in g1 + g2}
+    srv = mf.Server(ctx, hbm_budget_bytes=1 << 30, max_inflight_launches=2)
+    try:
+        ids1 = [(srv.submit(g), g) for g in g1]
+        t_end = time.time() + 0.003
+        while time.time() < t_end:
+            srv.progress()
+        ids2 = [(srv.submit(g), g) for g in g2]
+        pending = dict(ids1 + ids2)
+        t0 = time.time()
+        while pending and time.time() - t0 < 120:
+            srv.progress(wait=True)
+            for rid in list(pending):
+                r = srv.result(rid)
+                if r is not None:
+                    assert r[0] == expect[id(pending.pop(rid))]
+        assert not pending
+        t1 = [srv.times(rid) for rid, _ in ids1]
+        t2 = [srv.times(rid) for rid, _ in ids2]
+        assert all(t["batched"] == 1 for t in t1 + t2)
+        assert min(t["start_ms"] for t in t2) < max(t["finish_ms"] for t in t1)
+        st = srv.stats()
+        assert st["batched"] == 8 and st["serve_kernel_launches"] == 1
+        # the library's latency figures are the request times' percentiles
+        lat = np.array([t["finish_ms"] - t["submit_ms"] for t in t1 + t2])
+        p50, p99, sps = srv.latency()
+        assert abs(p50 - np.percentile(lat, 50)) < 1e-6 and abs(p99 - np.percentile(lat, 99)) < 1e-6
+        assert sps > 0
     finally:
         srv.close()

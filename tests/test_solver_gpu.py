@@ -135,6 +135,34 @@ def test_resolve_same_graph(mf, ctx, golden):
         dg.close()
 
 
+@pytest.mark.parametrize("graphs", [0, 1])
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_resolve_certified_every_layout(mf, orc, ctx, golden, layout, graphs):
+    # later solves of a resident graph restore the residuals on the device first (the solve after
+    # an upload uses the residuals the layout build wrote), and with cycle_graphs=1 they replay
+    # the cycle graphs captured by the first solve: each solve is certified
+    g, value = generated_graph(mf, golden, "rmat12_s3")
+    dg = ctx.upload(g, mf.LAYOUTS[layout])
+    replayed = 0
+    try:
+        for rep in range(3):
+            algo = mf.ALGO_VC if rep != 1 else mf.ALGO_TC
+            r = dg.solve(algo, cycle_graphs=graphs)
+            assert r.value == value, (layout, graphs, rep)
+            replayed += r.graph_launches
+            pre = dg.read_flow(return_excess=False)
+            cert, cut = orc.certify(g, pre, r.value, preflow=True)
+            assert cert == "ok", (layout, graphs, rep, cert, cut)
+            flow = dg.read_flow(return_excess=True)
+            cert, cut = orc.certify(g, flow, r.value)
+            assert cert == "ok" and cut == value, (layout, graphs, rep, cert, cut)
+            side = dg.read_cut()
+            assert cut_capacity(g, side) == value
+        assert (replayed > 0) == bool(graphs)
+    finally:
+        dg.close()
+
+
 @pytest.mark.parametrize("algo", ALGOS)
 def test_genrmf64_full_size(mf, orc, ctx, golden, algo):
     g, value = generated_graph(mf, golden, "genrmf64_s1")
@@ -142,9 +170,12 @@ def test_genrmf64_full_size(mf, orc, ctx, golden, algo):
     assert r.rounds >= 1 and r.pushes > 0
 
 
-def test_genrmf64_rcsr(mf, orc, ctx, golden):
+@pytest.mark.parametrize("layout", ["rcsr", "bcsr2"])
+@pytest.mark.parametrize("algo", ALGOS)
+def test_genrmf64_other_layouts(mf, orc, ctx, golden, algo, layout):
+    # Genrmf a=b=64 (262,144 vertices) on the layouts the bench reports as variants
     g, value = generated_graph(mf, golden, "genrmf64_s1")
-    solve_and_certify(mf, orc, ctx, g, "vc", "rcsr", value)
+    solve_and_certify(mf, orc, ctx, g, algo, layout, value)
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
@@ -331,6 +362,40 @@ def test_cli_output_contract(golden, algo):
                   r"implementation is (\d+), (\d+)", r.stdout)
     assert m and int(m.group(1)) == golden["fixtures"]["m50.max"]["value"]
     assert "Certificate: ok" in r.stdout
+
+
+@pytest.mark.parametrize("layout", ["0", "1", "2"])
+def test_cli_flow_and_cut_files(mf, orc, golden, tmp_path, layout):
+    # -o: the DIMACS solution lines of OutputFlow (network/DIMACS/washington/washington.c:1333-1399):
+    # "s <value>", then "f <tail> <head> <flow>" (1-based) for every arc with positive flow, in arc
+    # order; -x: "c cut <capacity>" and "n <vertex> s|t" for every vertex
+    g, value = generated_graph(mf, golden, "rlg32_s7")  # no parallel arcs: one line per (tail, head)
+    path = str(tmp_path / "g.max")
+    g.save_dimacs(path)
+    fout, xout = str(tmp_path / "flow.sol"), str(tmp_path / "cut.txt")
+    r = subprocess.run([os.path.join(ROOT, "wbpr_amd", "bin", "maxflow"), "-v", "2", "-f", path,
+                        "-a", "1", "-l", layout, "-o", fout, "-x", xout], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = open(fout).read().splitlines()
+    assert lines[0] == f"s {value}"
+    assert all(re.fullmatch(r"f \d+ \d+ [1-9]\d*", ln) for ln in lines[1:])
+    pos = {}
+    for ln in lines[1:]:
+        _, u, v, x = ln.split()
+        pos[(int(u) - 1, int(v) - 1)] = int(x)
+    tails = g.tails()
+    flow = np.array([pos.pop((int(u), int(v)), 0) for u, v in zip(tails, g.dst)], dtype=np.int32)
+    assert not pos  # every line names an arc of the graph
+    cert, cut = orc.certify(g, flow, value)
+    assert cert == "ok" and cut == value
+    cl = open(xout).read().splitlines()
+    assert cl[0] == f"c cut {value}" and len(cl) == g.n + 1
+    src_side = np.array([ln.split()[2] == "s" for ln in cl[1:]])
+    assert [ln.split()[1] for ln in cl[1:]] == [str(v + 1) for v in range(g.n)]
+    assert src_side[g.s] and not src_side[g.t]
+    cross = src_side[tails] & ~src_side[g.dst]
+    assert int(g.cap[cross].astype(np.int64).sum()) == value
 
 
 def test_cli_snap_with_terminals(golden):

@@ -165,6 +165,21 @@ class _ServerStats(C.Structure):
         ("launches", C.c_int64),
         ("arena_bytes", C.c_int64),
         ("arena_peak_bytes", C.c_int64),
+        ("batched", C.c_int64),
+        ("solo", C.c_int64),
+        ("serve_kernel_launches", C.c_int64),
+    ]
+
+
+class _ServerTimes(C.Structure):
+    _fields_ = [
+        ("submit_ms", C.c_double),
+        ("admit_ms", C.c_double),
+        ("start_ms", C.c_double),
+        ("finish_ms", C.c_double),
+        ("device_ms", C.c_double),
+        ("batched", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -224,6 +239,9 @@ _sig("mf_server_submit", C.c_int, _vp, C.POINTER(_Csr), C.POINTER(C.c_int64))
 _sig("mf_server_progress", C.c_int, _vp, C.c_int32)
 _sig("mf_server_result", C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double))
 _sig("mf_server_stats_get", C.c_int, _vp, C.POINTER(_ServerStats))
+_sig("mf_server_request_times", C.c_int, _vp, C.c_int64, C.POINTER(_ServerTimes))
+_sig("mf_server_latency", C.c_int, _vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+     C.POINTER(C.c_double))
 _sig("mf_server_destroy", C.c_int, _vp)
 _sig("mf_footprint_bytes", C.c_int64, C.c_int32, C.c_int64)
 _sig("mf_flow_return_excess", C.c_int, C.POINTER(_Csr), _i32p)
@@ -245,6 +263,7 @@ EXPORTED = [
     "mf_server_stats_get", "mf_server_destroy", "mf_footprint_bytes", "mf_flow_return_excess",
     "mf_autotune", "mf_batch_autotune", "mf_graph_read_layout", "mf_graph_read_workload",
     "mf_load_konect_bipartite", "mf_super_terminals", "mf_bfs_terminal_pairs",
+    "mf_server_request_times", "mf_server_latency",
 ]
 
 
@@ -686,6 +705,24 @@ class Server:
             return None
         _check(st, "server_result")
         return v.value, ms.value
+
+    def times(self, rid: int):
+        """Host times (ms since the server was created) of a completed request, or None."""
+        t = _ServerTimes()
+        st = _lib.mf_server_request_times(self._p, rid, C.byref(t))
+        if st == 11:
+            return None
+        _check(st, "server_request_times")
+        return {f: getattr(t, f) for f, _ in _ServerTimes._fields_ if f != "reserved"}
+
+    def latency(self):
+        """(p50 ms, p99 ms, solves/s) over the completed requests, or None before the first."""
+        p50, p99, sps = C.c_double(), C.c_double(), C.c_double()
+        st = _lib.mf_server_latency(self._p, C.byref(p50), C.byref(p99), C.byref(sps))
+        if st == 11:
+            return None
+        _check(st, "server_latency")
+        return p50.value, p99.value, sps.value
 
     def stats(self) -> dict:
         s = _ServerStats()

@@ -189,6 +189,11 @@ __device__ __forceinline__ V pick(const V (&a)[N], int i) {
   return v;
 }
 
+// Value of lane `src` of the wave (src the same in every lane, e.g. from a ballot).
+__device__ __forceinline__ int wave_lane(int v, int src) {
+  return __builtin_amdgcn_readlane(v, src);
+}
+
 // Number of items of class c, clamped to the region capacity.  The counter was completed by an
 // earlier kernel or, inside the one-workgroup BFS, lives in LDS behind a workgroup barrier.
 __device__ __forceinline__ int list_count(const ListSet &L, int c) {
@@ -219,7 +224,7 @@ __device__ __forceinline__ void append(const ListSet &L, int item, bool want, in
     const int rank = __popcll(mask & ((1ull << lane) - 1ull));
     int base = 0;
     if (lane == leader) base = atomicAdd(&L.cnt[c * L.cstride], __popcll(mask));
-    base = __shfl(base, leader);
+    base = wave_lane(base, leader);
     if (mine) {
       if (base + rank < L.cls[kNumClasses + c]) L.items[L.cls[c] + base + rank] = item;
       else *L.ovf = 1;
@@ -261,7 +266,7 @@ __device__ __forceinline__ void staged_append(Stage &S, const ListSet &L, int it
     const int rank = __popcll(mask & ((1ull << lane) - 1ull));
     int base = 0;
     if (lane == leader) base = atomicAdd(&S.cnt[c], __popcll(mask));
-    base = __shfl(base, leader);
+    base = wave_lane(base, leader);
     const bool fits = base + rank < kStageCap;
     if (mine && fits) S.items[c][base + rank] = item;
     const unsigned long long spill = __ballot(mine && !fits);
@@ -270,7 +275,7 @@ __device__ __forceinline__ void staged_append(Stage &S, const ListSet &L, int it
     const int srank = __popcll(spill & ((1ull << lane) - 1ull));
     int gbase = 0;
     if (lane == sl) gbase = atomicAdd(&L.cnt[c * L.cstride], __popcll(spill));
-    gbase = __shfl(gbase, sl);
+    gbase = wave_lane(gbase, sl);
     if (mine && !fits) {
       if (gbase + srank < L.cls[kNumClasses + c]) L.items[L.cls[c] + gbase + srank] = item;
       else *L.ovf = 1;
@@ -303,28 +308,165 @@ __device__ __forceinline__ unsigned long long pack_hk(int h, int key) {
   return ((unsigned long long)(unsigned)h << 32) | (unsigned)key;
 }
 
+// ---- cross-lane steps without the LDS crossbar ----------------------------------------------
+//
+// Lanes of a tile of T lanes (T a power of two dividing 64, tiles aligned in the wave) exchange
+// values through DPP lane modifiers and gfx950's permlane swaps, which are VALU operations,
+// instead of ds_bpermute / ds_swizzle, which go through the LDS unit and wait for it.
+//
+// Step OFF of a reduction gives every lane the value of one lane in the other half of its
+// aligned group of 2*OFF lanes:
+//   OFF 1, 2   quad permutations [1,0,3,2] and [2,3,0,1]            (DPP quad_perm)
+//   OFF 4      lane i <-> 7 - i in each group of 8                   (DPP row_half_mirror)
+//   OFF 8      lane i <-> 15 - i in each row of 16                   (DPP row_mirror)
+//   OFF 16     rows 0 <-> 1 and 2 <-> 3                              (v_permlane16_swap)
+//   OFF 32     lanes 0..31 <-> 32..63                                (v_permlane32_swap)
+// The mirrors are not lane ^ OFF, but after steps 1 .. OFF/2 every lane of a group of OFF
+// holds the group's reduction, so any partner in the other half completes the next level.
+// The permlane swaps return two registers that hold the lane's own value and its partner's;
+// the reductions below only combine them with commutative operations.
+enum : int {
+  kDppQuadXor1 = 0xB1,  // quad_perm [1,0,3,2]
+  kDppQuadXor2 = 0x4E,  // quad_perm [2,3,0,1]
+  kDppRowShr = 0x110,   // + n: row_shr:n (n = 1..15)
+  kDppRowMirror = 0x140,
+  kDppRowHalfMirror = 0x141,
+  kDppRowBcast15 = 0x142,
+  kDppRowBcast31 = 0x143,
+};
+
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int dpp(int x) {
+  // lanes whose row is masked off, or whose source lane is outside the row, receive 0
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, ROW_MASK, 0xF, true);
+}
+
+// The two registers {own, partner} of reduction step OFF (see above), for 32-bit x.
+template <int OFF>
+__device__ __forceinline__ void xpair(int x, int &p0, int &p1) {
+  if constexpr (OFF == 1) {
+    p0 = x;
+    p1 = dpp<kDppQuadXor1>(x);
+  } else if constexpr (OFF == 2) {
+    p0 = x;
+    p1 = dpp<kDppQuadXor2>(x);
+  } else if constexpr (OFF == 4) {
+    p0 = x;
+    p1 = dpp<kDppRowHalfMirror>(x);
+  } else if constexpr (OFF == 8) {
+    p0 = x;
+    p1 = dpp<kDppRowMirror>(x);
+  } else if constexpr (OFF == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    p0 = (int)r[0];
+    p1 = (int)r[1];
+  } else {
+    static_assert(OFF == 32, "reduction steps are 1, 2, 4, 8, 16, 32 lanes");
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    p0 = (int)r[0];
+    p1 = (int)r[1];
+  }
+}
+
+// 64-bit values: both halves take the same route, so each returned pair is one lane's value.
+template <int OFF>
+__device__ __forceinline__ void xpair64(unsigned long long x, unsigned long long &p0,
+                                        unsigned long long &p1) {
+  int l0, l1, h0, h1;
+  xpair<OFF>((int)(unsigned)x, l0, l1);
+  xpair<OFF>((int)(unsigned)(x >> 32), h0, h1);
+  p0 = ((unsigned long long)(unsigned)h0 << 32) | (unsigned)l0;
+  p1 = ((unsigned long long)(unsigned)h1 << 32) | (unsigned)l1;
+}
+
 template <int T>
 __device__ __forceinline__ unsigned long long tile_min(unsigned long long v) {
-#pragma unroll
-  for (int off = T / 2; off > 0; off >>= 1) {
-    unsigned long long o = __shfl_xor(v, off, T);
-    v = o < v ? o : v;
+  static_assert(T >= 1 && T <= 64 && (T & (T - 1)) == 0, "tile of 1..64 lanes");
+  if constexpr (T > 1) {
+    unsigned long long a, b;
+    xpair64<T / 2>(tile_min<T / 2>(v), a, b);
+    return a < b ? a : b;
   }
   return v;
 }
 
 template <int T>
 __device__ __forceinline__ int tile_max(int v) {
-#pragma unroll
-  for (int off = T / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, T));
+  static_assert(T >= 1 && T <= 64 && (T & (T - 1)) == 0, "tile of 1..64 lanes");
+  if constexpr (T > 1) {
+    int a, b;
+    xpair<T / 2>(tile_max<T / 2>(v), a, b);
+    return max(a, b);
+  }
+  return v;
+}
+
+template <int T>
+__device__ __forceinline__ int tile_sum32(int v) {
+  static_assert(T >= 1 && T <= 64 && (T & (T - 1)) == 0, "tile of 1..64 lanes");
+  if constexpr (T > 1) {
+    int a, b;
+    xpair<T / 2>(tile_sum32<T / 2>(v), a, b);
+    return a + b;
+  }
   return v;
 }
 
 template <int T>
 __device__ __forceinline__ long long tile_sum(long long v) {
-#pragma unroll
-  for (int off = T / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, T);
+  static_assert(T >= 1 && T <= 64 && (T & (T - 1)) == 0, "tile of 1..64 lanes");
+  if constexpr (T > 1) {
+    unsigned long long a, b;
+    xpair64<T / 2>((unsigned long long)tile_sum<T / 2>(v), a, b);
+    return (long long)(a + b);
+  }
   return v;
+}
+
+// Value of the tile's first lane, in every lane of the tile.
+template <int T>
+__device__ __forceinline__ int tile_first(int v) {
+  if constexpr (T == 1) {
+    return v;
+  } else if constexpr (T == 64) {
+    return __builtin_amdgcn_readlane(v, 0);
+  } else {
+    return tile_sum32<T>((__lane_id() & (T - 1)) == 0 ? v : 0);
+  }
+}
+
+// One step of an inclusive scan: lanes whose position in their row of 16 is >= N add the value
+// N lanes below them.
+template <int N>
+__device__ __forceinline__ unsigned long long scan_row_step(unsigned long long v) {
+  const int lo = dpp<kDppRowShr + N>((int)(unsigned)v);
+  const int hi = dpp<kDppRowShr + N>((int)(unsigned)(v >> 32));
+  const unsigned long long y = ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+  return (__lane_id() & 15) >= N ? v + y : v;
+}
+
+// Inclusive prefix sum over the lanes of each tile of T lanes, in lane order: row shifts inside
+// each row of 16, then the total of row 0 (2) broadcast into row 1 (3), then the total of lanes
+// 0..31 into lanes 32..63 (DPP row_bcast).
+template <int T>
+__device__ __forceinline__ long long tile_scan(long long sv) {
+  static_assert(T >= 1 && T <= 64 && (T & (T - 1)) == 0, "tile of 1..64 lanes");
+  unsigned long long v = (unsigned long long)sv;
+  if constexpr (T >= 2) v = scan_row_step<1>(v);
+  if constexpr (T >= 4) v = scan_row_step<2>(v);
+  if constexpr (T >= 8) v = scan_row_step<4>(v);
+  if constexpr (T >= 16) v = scan_row_step<8>(v);
+  if constexpr (T >= 32) {
+    const int lo = dpp<kDppRowBcast15, 0xA>((int)(unsigned)v);
+    const int hi = dpp<kDppRowBcast15, 0xA>((int)(unsigned)(v >> 32));
+    v += ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+  }
+  if constexpr (T >= 64) {
+    const int lo = dpp<kDppRowBcast31, 0xC>((int)(unsigned)v);
+    const int hi = dpp<kDppRowBcast31, 0xC>((int)(unsigned)(v >> 32));
+    v += ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+  }
+  return (long long)v;
 }
 
 // Statistics accumulated per thread and flushed once per block.

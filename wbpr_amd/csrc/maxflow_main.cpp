@@ -8,7 +8,9 @@
 //   "Total kernel time: %.6f ms"
 //   "The maximum flow value of this flow network as calculated by the parallel implementation is %d, %d"
 // Extra options of this build: -l residual layout (0 RCSR, 1 BCSR, 2 BCSR2; default 1),
-// -k push-relabel cycles per global relabel, -c certificate check, -o DIMACS flow output file.
+// -k push-relabel cycles per global relabel, -c certificate check, -o DIMACS flow output file
+// (the s/f lines of OutputFlow, network/DIMACS/washington/washington.c:1333-1399), -x minimum-cut
+// output file.
 #include <getopt.h>
 #include <unistd.h>
 
@@ -39,6 +41,7 @@ static void usage(const char *prog) {
   printf("\t-k cycles\tPush-relabel cycles between global relabels (default automatic)\n");
   printf("\t-c\t\tCheck the result: capacity, conservation and min-cut certificate\n");
   printf("\t-o file\t\tWrite the flow in DIMACS solution format (s/f lines)\n");
+  printf("\t-x file\t\tWrite a minimum cut: 'c cut <capacity>', then 'n <vertex> s|t' per vertex\n");
   printf("\t-w\t\tPrint the busy cycles of every wave of the push-relabel launches\n");
   printf("\t-m pairs\tSNAP input without -s/-t: super-source and super-sink over BFS-chosen pairs\n");
 }
@@ -112,9 +115,9 @@ static int check_certificate(const mf_csr *c, const int32_t *flow, long long val
 int main(int argc, char **argv) {
   int ch, file_type = 0, algo_type = 0, layout = 1, cycles = 0, check = 0, workload = 0, pairs = 0;
   int source = -1, sink = -1;
-  const char *filename = nullptr, *outfile = nullptr;
+  const char *filename = nullptr, *outfile = nullptr, *cutfile = nullptr;
   mf_host_graph *g = nullptr;
-  while ((ch = getopt(argc, argv, "hv:f:s:t:a:l:k:co:wm:")) != -1) {
+  while ((ch = getopt(argc, argv, "hv:f:s:t:a:l:k:co:x:wm:")) != -1) {
     switch (ch) {
       case 'h':
         usage(argv[0]);
@@ -174,6 +177,9 @@ int main(int argc, char **argv) {
         break;
       case 'o':
         outfile = optarg;
+        break;
+      case 'x':
+        cutfile = optarg;
         break;
       case 'w':
         workload = 1;
@@ -271,7 +277,7 @@ int main(int argc, char **argv) {
     for (int64_t i = 0; i < nw; ++i) printf("%llu ", (unsigned long long)cyc[(size_t)i]);
     printf("\n");
   }
-  if (check || outfile) {
+  if (check || outfile || cutfile) {
     std::vector<int32_t> flow((size_t)c.m);
     std::vector<uint8_t> side((size_t)c.n);
     if (mf_graph_read_flow(ctx, dg, flow.data(), 1) != MF_OK ||
@@ -290,6 +296,21 @@ int main(int argc, char **argv) {
           for (int u = 0; u < c.n; ++u)
             for (int i = c.offsets[u]; i < c.offsets[u + 1]; ++i)
               if (flow[(size_t)i] > 0) fprintf(f, "f %d %d %d\n", u + 1, c.dst[i] + 1, flow[(size_t)i]);
+          fclose(f);
+        }
+      }
+      if (cutfile) {  // source side: the vertices that cannot reach t in the residual graph
+        FILE *f = fopen(cutfile, "w");
+        if (!f) {
+          fprintf(stderr, "Cannot write %s\n", cutfile);
+          rc = 1;
+        } else {
+          long long cut = 0;
+          for (int u = 0; u < c.n; ++u)
+            for (int i = c.offsets[u]; i < c.offsets[u + 1]; ++i)
+              if (!side[(size_t)u] && side[(size_t)c.dst[i]]) cut += c.cap ? c.cap[i] : 1;
+          fprintf(f, "c cut %lld\n", cut);
+          for (int u = 0; u < c.n; ++u) fprintf(f, "n %d %c\n", u + 1, side[(size_t)u] ? 't' : 's');
           fclose(f);
         }
       }

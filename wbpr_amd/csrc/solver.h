@@ -56,4 +56,43 @@ mf_status batch_upload(mf_context *ctx, const mf_csr *packed, const int32_t *src
 mf_status batch_solve_resident(mf_context *ctx, BatchResident *r, const mf_options &o,
                                int64_t *values, mf_result *res);
 void batch_release(BatchResident *r);
+
+// ---- serving arena: a device arena of vertex and slot ranges, and a persistent serving kernel
+// that solves the instances the host publishes into a queue while it runs (k_serve_wg) ----
+struct ServeArena;
+// A request laid out for the arena on the host: BCSR slots with local ids (offsets n+1,
+// {dst, capacity} pairs, reverse slots), vertex BFS classes and the class populations.  The
+// a_* arrays hold the same data in arena ids while its copy is in flight.
+struct ServeInstance {
+  int32_t n = 0, s = 0, t = 0;
+  int64_t S = 0;
+  std::vector<int32_t> off, dc, rev, a_off, a_dc, a_rev;
+  std::vector<unsigned char> vcls;
+  int32_t pop[kClasses], bpop[kClasses];
+};
+mf_status serve_prepare(const mf_csr *g, ServeInstance &inst);
+// Device bytes of an arena of nv vertices and ns slots (an instance takes n + 1 vertices and
+// 2m slots).
+int64_t serve_arena_bytes(int64_t nv, int64_t ns);
+mf_status serve_arena_create(mf_context *ctx, int64_t nv, int64_t ns, int32_t qcap,
+                             ServeArena **out);
+// Leaves idle workgroups, waits for the kernel and the copies, frees everything.
+void serve_arena_destroy(ServeArena *a);
+// Queues the copies of inst into vertices [vb, vb + n] and slots [sb, sb + S) of the arena;
+// *done is a ticket for serve_copy_state / serve_copy_release.  inst must stay alive (and its
+// a_* arrays unchanged) until the copy has finished.
+mf_status serve_copy(ServeArena *a, ServeInstance &inst, int32_t vb, int64_t sb, void **done);
+int serve_copy_state(void *done);
+void serve_copy_release(void *done);
+// Publishes queue entry q (q = 0, 1, 2, ... in order; at most qcap unreported entries).
+void serve_publish(ServeArena *a, int64_t q, const ServeInstance &inst, int32_t vb);
+// True once entry q is reported: its value, status (0 ok, 1 round limit, 2 invariant broken)
+// and the device time of its solve.
+bool serve_poll(ServeArena *a, int64_t q, int64_t *value, int32_t *status, double *device_ms);
+// Whether the serving kernel is still running; launching it (again) when it is not.
+bool serve_running(ServeArena *a);
+mf_status serve_launch(ServeArena *a);
+// The device a context runs on.
+int context_device(const mf_context *ctx);
+int64_t serve_launches(const ServeArena *a);
 }  // namespace mf

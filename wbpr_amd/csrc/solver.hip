@@ -152,17 +152,6 @@ __device__ __forceinline__ int slot_degree(const L &g, int u) {
 template <int T>
 constexpr int scan_batch() { return (T == 1 || T > 64) ? 8 : 4; }
 
-template <int T>
-__device__ __forceinline__ long long tile_scan(long long v) {
-  const int lane = __lane_id() % T;
-#pragma unroll
-  for (int off = 1; off < T; off <<= 1) {
-    const long long y = __shfl_up(v, off, T);
-    if (lane >= off) v += y;
-  }
-  return v;
-}
-
 // What the group decided for u after the min-reduction.
 struct Plan {
   bool dead;   // no residual arc to a reached vertex: u can never reach a sink again
@@ -274,9 +263,9 @@ __device__ __forceinline__ int discharge_tile(const L &g, const CycleArgs &a, St
   const int hu = a.h[u];
   const int e0 = lane == 0 ? a.e[u] : 0;
   const int2 r0 = g.range(u, 0);
-  if (claim && __shfl(old, 0, T) == cycle_now(a)) return -1;  // discharged in this cycle already
+  if (claim && tile_first<T>(old) == cycle_now(a)) return -1;  // discharged in this cycle already
   // e(u) grows while the tile runs: every lane uses lane 0's read (the value subtracted below)
-  const int eu = __shfl(e0, 0, T);
+  const int eu = tile_first<T>(e0);
   if (hu >= a.H) return -1;  // became inactive after it was queued (all lanes read the same value)
   Cand c;
   int deg;
@@ -332,11 +321,10 @@ template <int T>
 struct Group {
   __device__ static unsigned long long min(unsigned long long v) { return tile_min<T>(v); }
   __device__ static long long scan(long long v, long long &total) {
-    const long long incl = tile_scan<T>(v);
-    total = __shfl(incl, T - 1, T);
-    return incl;
+    total = tile_sum<T>(v);
+    return tile_scan<T>(v);
   }
-  __device__ static int first(int v) { return __shfl(v, 0, T); }  // lane 0's value
+  __device__ static int first(int v) { return tile_first<T>(v); }  // lane 0's value
 };
 template <>
 struct Group<kBlock> {
@@ -464,7 +452,7 @@ __device__ __forceinline__ bool claim_queued(const CycleArgs &a, int u, int lane
   int old = 0;
   const int cyc = cycle_now(a);
   if (lane == 0) old = atomicExch(&a.stamp[u], cyc);
-  return __shfl(old, 0, T) != cyc;
+  return tile_first<T>(old) != cyc;
 }
 
 // claim: u was taken from the worklist while forwarding is on (stamp it first, claim_queued).
@@ -1181,7 +1169,7 @@ __device__ __forceinline__ void wg_append(const WgOut &o, int w, bool want, int 
     const int rank = __popcll(mask & ((1ull << lane) - 1ull));
     int base = 0;
     if (lane == leader) base = atomicAdd(&o.n[q], __popcll(mask));
-    base = __shfl(base, leader);
+    base = wave_lane(base, leader);
     if (mine) {
       const int pos = base + rank;
       if (pos < wg_cap(q)) {
@@ -1529,18 +1517,28 @@ __device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &c
   }
 }
 
+// One instance of a batch: vertices [vb, vb + n), source s and sink t (ids in the packed
+// space), and its class populations (cycle classes, BFS classes; kNumClasses each).
+struct BatchInst {
+  int vb, n, s, t;
+  const int *pop, *bpop;
+};
+
+// Solves one instance inside the calling workgroup; returns its value (the sink's excess) and
+// *status_out (0 ok, 1 round limit, 2 a source was reached: invariant broken).
 template <class L>
-__device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, int inst,
+__device__ __forceinline__ long long batch_instance(const L &g, const BatchArgs &b,
+                               const BatchInst &I, int *status_out,
                                int (*s_f)[kNumClasses], int (*s_c)[kNumClasses], int *s_flag,
                                int *s_dec, int (*s_tab)[2 * kNumClasses], Stats &st,
                                long long &nv, long long &ns) {
   const int tid = (int)threadIdx.x;
-  const int vb = b.vbase[inst], n = b.nver[inst], s = b.src[inst], t = b.snk[inst];
+  const int vb = I.vb, n = I.n, s = I.s, t = I.t;
   const int H = n;  // heights are distances inside the instance
   // class regions of the instance's frontier lists (table 0) and cycle worklists (table 1) in
   // the packed list arrays, in LDS
   if (tid < 2) {
-    const int *pp = (tid == 0 ? b.bpop : b.pop) + (size_t)inst * kNumClasses;
+    const int *pp = tid == 0 ? I.bpop : I.pop;
     const int nc = tid == 0 ? kNumClasses : b.nclass;
     int acc = vb;
     for (int c = 0; c < kNumClasses; ++c) {
@@ -1689,11 +1687,10 @@ __device__ __forceinline__ void batch_instance(const L &g, const BatchArgs &b, i
       if (stop) break;
     }
   }
-  if (tid == 0) {
-    b.values[inst] = b.e[t];
-    b.status[inst] = status;
-  }
+  const long long value = b.e[t];  // every thread: the last cycle's atomics are behind a barrier
+  *status_out = status;
   __syncthreads();
+  return value;
 }
 
 template <class L>
@@ -1712,10 +1709,159 @@ __global__ __launch_bounds__(kBlock) void k_batch_wg(L g, BatchArgs b) {
     const int i = s_inst;
     __syncthreads();
     if (i >= b.k) break;
-    batch_instance(g, b, b.order[i], s_f, s_c, s_flag, &s_dec, s_tab, st, nv, ns);
+    const int inst = b.order[i];
+    const BatchInst I{b.vbase[inst], b.nver[inst], b.src[inst], b.snk[inst],
+                      b.pop + (size_t)inst * kNumClasses, b.bpop + (size_t)inst * kNumClasses};
+    int status = 0;
+    const long long value = batch_instance(g, b, I, &status, s_f, s_c, s_flag, &s_dec, s_tab, st, nv, ns);
+    if (threadIdx.x == 0) {
+      b.values[inst] = value;
+      b.status[inst] = status;
+    }
   }
   block_flush(st.active, st.slots, st.pushes, st.relabels, b.stats, ST_ACTIVE);
   block_flush(nv, ns, 0, 0, b.stats, ST_BFS_VERTS);
+}
+
+// ---- serving: instances admitted while the kernel runs ---------------------------------------
+//
+// A persistent grid that serves a queue the host keeps filling.  The host copies an admitted
+// request's residual slots, offsets and vertex classes into its ranges of the server's device
+// arena, then publishes queue entry q: it writes the entry (vertex range, terminals, class
+// populations) and then the published count, both in host memory that the GPU reads uncached.
+// A workgroup claims the next published entry on a device counter, solves the instance exactly
+// like a batched one (batch_instance), and writes the value, status and device times into the
+// entry's report in host memory, the entry number last.  An idle workgroup sleeps between looks
+// at the published count and leaves once it has been idle for idle_ticks, or when the host closes
+// the queue, so the grid drains by itself; the host launches it again when entries are published
+// that no running workgroup will take.
+//
+// Arena ranges are reused by later requests, and those copies happen while a workgroup may hold
+// lines of the range in its caches: a workgroup invalidates its caches (agent-scope acquire)
+// before it reads an admitted instance, and writes its dirty lines back (agent-scope release)
+// before it reports the instance, after which the host may copy a new request over the range.
+// Idle workgroups read the host's published count at most once per kHostPollTicks between them
+// (the last value read is kept in device memory), so the grid does not flood the host link.
+
+struct ServeEntry {
+  int vb, n, s, t;        // vertex range [vb, vb + n) in the arena, source, sink
+  int pop[kNumClasses];   // cycle-class populations of the instance
+  int bpop[kNumClasses];  // BFS-class populations
+};
+constexpr int kServeEntryInts = 4 + 2 * kNumClasses;
+static_assert(sizeof(ServeEntry) == sizeof(int) * kServeEntryInts, "ServeEntry is plain ints");
+
+struct ServeReport {
+  long long value;
+  long long t_start, t_end;  // wall_clock64 ticks (100 MHz) around the instance's solve
+  int status;                // 0 ok, 1 round limit, 2 invariant broken
+  int seq;                   // entry number + 1, written last
+};
+
+struct ServeArgs {
+  BatchArgs b;                 // arena arrays and solver options (per-instance fields unused)
+  const int *entries;          // host memory: qcap entries of kServeEntryInts ints
+  ServeReport *reports;        // host memory: qcap reports
+  const int *published;        // host memory: entries published so far
+  const int *closed;           // host memory: nonzero asks idle workgroups to leave
+  int *claimed;                // device memory: entries claimed so far
+  int *pub_seen;               // device memory: the largest published count read so far
+  long long *poll_at;          // device memory: wall_clock64 of the last host read
+  int qcap;
+  long long idle_ticks;        // ticks an idle workgroup waits for work before it leaves
+};
+constexpr long long kHostPollTicks = 500;  // 5 us at the 100 MHz wall clock
+
+// Published entries, as one idle workgroup's thread sees them: the host's count when no
+// workgroup has read it for kHostPollTicks, else the last count read.
+__device__ __forceinline__ int serve_published(const ServeArgs &a) {
+  const long long now = (long long)wall_clock64();
+  unsigned long long last =
+      __hip_atomic_load((unsigned long long *)a.poll_at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (now - (long long)last >= kHostPollTicks &&
+      atomicCAS((unsigned long long *)a.poll_at, last, (unsigned long long)now) == last) {
+    const int pub = __hip_atomic_load(a.published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_fetch_max(a.pub_seen, pub, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    return pub;
+  }
+  return __hip_atomic_load(a.pub_seen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_serve_wg(L g, ServeArgs a) {
+  __shared__ int s_f[3][kNumClasses];
+  __shared__ int s_c[3][kNumClasses];
+  __shared__ int s_flag[2];
+  __shared__ int s_dec;
+  __shared__ int s_q;
+  __shared__ int s_tab[2][2 * kNumClasses];
+  __shared__ int s_entry[kServeEntryInts];
+  const int tid = (int)threadIdx.x;
+  Stats st{0, 0, 0, 0};
+  long long nv = 0, ns = 0;
+  long long idle_from = (long long)wall_clock64();
+  for (;;) {
+    if (tid == 0) {
+      int q = -1;
+      const int pub = serve_published(a);
+      int c = __hip_atomic_load(a.claimed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      while (c < pub) {  // every failed exchange means another workgroup claimed entry c
+        const int old = atomicCAS(a.claimed, c, c + 1);
+        if (old == c) {
+          q = c;
+          break;
+        }
+        c = old;
+      }
+      if (q < 0 && ((long long)wall_clock64() - idle_from > a.idle_ticks ||
+                    __hip_atomic_load(a.closed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)))
+        q = -2;
+      s_q = q;
+    }
+    __syncthreads();
+    const int q = s_q;
+    __syncthreads();
+    if (q == -2) break;
+    if (q == -1) {
+      __builtin_amdgcn_s_sleep(127);
+      __builtin_amdgcn_s_sleep(127);
+      continue;
+    }
+    const long long t0 = (long long)wall_clock64();
+    const int slot = q % a.qcap;
+    if (tid < kServeEntryInts)
+      s_entry[tid] = __hip_atomic_load(a.entries + (size_t)slot * kServeEntryInts + tid,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    // the request's arena ranges were written by a copy after this kernel may have cached them
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __syncthreads();
+    const BatchInst I{s_entry[0], s_entry[1], s_entry[2], s_entry[3], s_entry + 4,
+                      s_entry + 4 + kNumClasses};
+    // host state of the instance's vertices: no excess, never discharged
+    for (int v = I.vb + tid; v < I.vb + I.n; v += kBlock) {
+      a.b.e[v] = 0;
+      a.b.stamp[v] = -1;
+    }
+    __syncthreads();
+    int status = 0;
+    const long long value =
+        batch_instance(g, a.b, I, &status, s_f, s_c, s_flag, &s_dec, s_tab, st, nv, ns);
+    // every write of the instance reaches memory before the host may reuse its ranges
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) {
+      ServeReport *r = a.reports + slot;
+      __hip_atomic_store(&r->value, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&r->t_start, t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&r->t_end, (long long)wall_clock64(), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&r->status, status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&r->seq, q + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    idle_from = (long long)wall_clock64();
+  }
+  block_flush(st.active, st.slots, st.pushes, st.relabels, a.b.stats, ST_ACTIVE);
+  block_flush(nv, ns, 0, 0, a.b.stats, ST_BFS_VERTS);
 }
 
 }  // namespace mf
@@ -1866,6 +2012,7 @@ struct mf_context {
   char name[256] = {0};
   char arch[64] = {0};
   std::vector<hipEvent_t> prof_ev;
+  hipEvent_t ev_solve[2] = {nullptr, nullptr};  // brackets of the solve loop's phases
 };
 
 namespace mf {
@@ -2281,9 +2428,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   const int vgrid = std::max(1, std::min((n + 255) / 256, 16 * ctx->num_cu));
   const int kgrid = std::max(1, std::min((G.k + 255) / 256, 16 * ctx->num_cu));
 
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
+  hipEvent_t e0 = ctx->ev_solve[0], e1 = ctx->ev_solve[1];
   double kernel_ms = 0, relabel_ms = 0, prof_ms = 0;
   int64_t prof_launches = 0, cycles = 0, rounds = 0, bfs_levels = 0, bfs_grid_levels = 0,
           bfs_bu_levels = 0;
@@ -2528,8 +2673,6 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   long long v0 = 0;
   HIP_TRY(hipMemcpy(&v0, G.values, sizeof(long long), hipMemcpyDeviceToHost));
   HIP_TRY(read_stats(G.stats, hs));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (res) {
     memset(res, 0, sizeof(*res));
     res->value = v0;
@@ -2797,7 +2940,248 @@ void batch_release(BatchResident *r) {
   delete r;
 }
 
+// ---- serving arena (host side of k_serve_wg) ---------------------------------------------------
+
+constexpr int kServeGroupsPerCU = 1;               // resident serving workgroups per CU
+constexpr long long kServeIdleTicks = 200000;      // 2 ms at the 100 MHz wall clock
+
+struct ServeArena {
+  mf_context *ctx = nullptr;
+  int64_t nv = 0, ns = 0;  // vertex and slot capacity
+  int32_t qcap = 0;
+  // device arena
+  int *off = nullptr, *rev = nullptr, *h = nullptr, *e = nullptr, *stamp = nullptr;
+  int *lists = nullptr;
+  int2 *dc = nullptr;
+  unsigned char *vcls = nullptr;
+  int *words = nullptr;  // [0] claimed, [1] published count seen, [2] overflow, [4..5] poll time
+  unsigned long long *stats = nullptr;
+  // host memory the GPU reads and writes uncached
+  int *entries = nullptr;
+  ServeReport *reports = nullptr;
+  int *pub = nullptr;  // [0] published count, [1] closed
+  hipStream_t copy_stream = nullptr, kernel_stream = nullptr;
+  bool launched = false;
+  int64_t launches = 0;
+};
+
+void serve_arena_destroy(ServeArena *a) {
+  if (!a) return;
+  (void)hipSetDevice(a->ctx->device);
+  if (a->pub) __atomic_store_n(&a->pub[1], 1, __ATOMIC_RELEASE);  // idle workgroups leave
+  if (a->kernel_stream) (void)hipStreamSynchronize(a->kernel_stream);
+  if (a->copy_stream) (void)hipStreamSynchronize(a->copy_stream);
+  for (void *p : {(void *)a->off, (void *)a->rev, (void *)a->h, (void *)a->e, (void *)a->stamp,
+                  (void *)a->lists, (void *)a->dc, (void *)a->vcls, (void *)a->words,
+                  (void *)a->stats})
+    if (p) (void)hipFree(p);
+  for (void *p : {(void *)a->entries, (void *)a->reports, (void *)a->pub})
+    if (p) (void)hipHostFree(p);
+  if (a->copy_stream) (void)hipStreamDestroy(a->copy_stream);
+  if (a->kernel_stream) (void)hipStreamDestroy(a->kernel_stream);
+  delete a;
+}
+
+int64_t serve_arena_bytes(int64_t nv, int64_t ns) {
+  // off 4, h 4, e 4, stamp 4, class 1, four lists 16 per vertex; {dst, residual} 8 + reverse 4
+  // per slot
+  return 33 * nv + 12 * ns;
+}
+
+mf_status serve_arena_create(mf_context *ctx, int64_t nv, int64_t ns, int32_t qcap,
+                             ServeArena **out) {
+  if (!ctx || !out || nv <= 0 || ns <= 0 || qcap <= 0 || nv >= INT32_MAX / 4 || ns >= INT32_MAX)
+    return MF_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  ServeArena *a = new ServeArena();
+  a->ctx = ctx;
+  a->nv = nv;
+  a->ns = ns;
+  a->qcap = qcap;
+  mf_status rc = MF_OK;
+  auto fail = [&](mf_status r) {
+    serve_arena_destroy(a);
+    return r;
+  };
+  if ((rc = dmalloc(&a->off, (size_t)nv + 1)) || (rc = dmalloc(&a->rev, (size_t)ns)) ||
+      (rc = dmalloc(&a->h, (size_t)nv)) || (rc = dmalloc(&a->e, (size_t)nv)) ||
+      (rc = dmalloc(&a->stamp, (size_t)nv)) || (rc = dmalloc(&a->lists, (size_t)4 * nv)) ||
+      (rc = dmalloc(&a->dc, (size_t)ns)) || (rc = dmalloc(&a->vcls, (size_t)nv)) ||
+      (rc = dmalloc(&a->words, (size_t)64)) ||
+      (rc = dmalloc(&a->stats, (size_t)kStatSlot * kStatSpread)))
+    return fail(rc);
+  if (hipHostMalloc((void **)&a->entries, sizeof(int) * kServeEntryInts * (size_t)qcap,
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostMalloc((void **)&a->reports, sizeof(ServeReport) * (size_t)qcap,
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostMalloc((void **)&a->pub, sizeof(int) * 16, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+    return fail(MF_ERR_OUT_OF_MEMORY);
+  memset(a->entries, 0, sizeof(int) * kServeEntryInts * (size_t)qcap);
+  memset((void *)a->reports, 0, sizeof(ServeReport) * (size_t)qcap);
+  memset(a->pub, 0, sizeof(int) * 16);
+  if (hipStreamCreateWithFlags(&a->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&a->kernel_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMemsetAsync(a->words, 0, sizeof(int) * 64, a->kernel_stream) != hipSuccess ||
+      hipMemsetAsync(a->stats, 0, sizeof(unsigned long long) * kStatSlot * kStatSpread,
+                     a->kernel_stream) != hipSuccess ||
+      hipStreamSynchronize(a->kernel_stream) != hipSuccess)
+    return fail(MF_ERR_DEVICE);
+  *out = a;
+  return MF_OK;
+}
+
+mf_status serve_prepare(const mf_csr *g, ServeInstance &I) {
+  mf_status st = validate_csr(g);
+  if (st != MF_OK) return st;
+  SlotLayout L;
+  if ((st = build_slot_layout(g, L)) != MF_OK) return st;
+  const ClassConfig vc = class_config(MF_ALGO_VC, 0);
+  I.n = g->n;
+  I.S = L.slots;
+  I.s = g->s;
+  I.t = g->t;
+  I.off = std::move(L.off);
+  I.rev = std::move(L.rev);
+  I.dc.resize((size_t)(2 * I.S));
+  for (int64_t j = 0; j < I.S; ++j) {
+    I.dc[(size_t)(2 * j)] = L.dst[(size_t)j];
+    I.dc[(size_t)(2 * j + 1)] = L.cap0[(size_t)j];
+  }
+  I.vcls.resize((size_t)I.n);
+  for (int q = 0; q < kNumClasses; ++q) I.pop[q] = I.bpop[q] = 0;
+  for (int32_t v = 0; v < I.n; ++v) {
+    const int c = class_of_degree(vc, (int64_t)(I.off[(size_t)v + 1] - I.off[(size_t)v]));
+    I.vcls[(size_t)v] = (unsigned char)c;
+    I.bpop[c]++;
+  }
+  I.pop[0] = I.n;  // thread-centric cycles: one class
+  return MF_OK;
+}
+
+mf_status serve_copy(ServeArena *a, ServeInstance &I, int32_t vb, int64_t sb, void **done) {
+  if (vb < 0 || (int64_t)vb + I.n + 1 > a->nv || sb < 0 || sb + I.S > a->ns)
+    return MF_ERR_INVALID_ARGUMENT;
+  HIP_TRY(hipSetDevice(a->ctx->device));
+  // arena ids: vertices from vb, slots from sb
+  I.a_off.resize((size_t)I.n + 1);
+  for (int32_t v = 0; v <= I.n; ++v) I.a_off[(size_t)v] = (int32_t)(sb + I.off[(size_t)v]);
+  I.a_dc.resize(I.dc.size());
+  I.a_rev.resize(I.rev.size());
+  for (int64_t j = 0; j < I.S; ++j) {
+    I.a_dc[(size_t)(2 * j)] = vb + I.dc[(size_t)(2 * j)];
+    I.a_dc[(size_t)(2 * j + 1)] = I.dc[(size_t)(2 * j + 1)];
+    I.a_rev[(size_t)j] = (int32_t)(sb + I.rev[(size_t)j]);
+  }
+  hipStream_t st = a->copy_stream;
+  HIP_TRY(hipMemcpyAsync(a->off + vb, I.a_off.data(), sizeof(int) * ((size_t)I.n + 1),
+                         hipMemcpyHostToDevice, st));
+  if (I.S) {
+    HIP_TRY(hipMemcpyAsync(a->dc + sb, I.a_dc.data(), sizeof(int2) * (size_t)I.S,
+                           hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(a->rev + sb, I.a_rev.data(), sizeof(int) * (size_t)I.S,
+                           hipMemcpyHostToDevice, st));
+  }
+  HIP_TRY(hipMemcpyAsync(a->vcls + vb, I.vcls.data(), (size_t)I.n, hipMemcpyHostToDevice, st));
+  hipEvent_t ev = nullptr;
+  HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  if (hipEventRecord(ev, st) != hipSuccess) {
+    (void)hipEventDestroy(ev);
+    return MF_ERR_DEVICE;
+  }
+  *done = ev;
+  return MF_OK;
+}
+
+// 1: the copies behind the ticket have finished, 0: not yet, -1: the copy failed.
+int serve_copy_state(void *done) {
+  const hipError_t e = hipEventQuery((hipEvent_t)done);
+  return e == hipSuccess ? 1 : (e == hipErrorNotReady ? 0 : -1);
+}
+
+void serve_copy_release(void *done) {
+  if (done) (void)hipEventDestroy((hipEvent_t)done);
+}
+
+void serve_publish(ServeArena *a, int64_t q, const ServeInstance &I, int32_t vb) {
+  int *en = a->entries + (size_t)(q % a->qcap) * kServeEntryInts;
+  const int vals[kServeEntryInts] = {vb, I.n, vb + I.s, vb + I.t,
+                                     I.pop[0], I.pop[1], I.pop[2], I.pop[3], I.pop[4], I.pop[5],
+                                     I.bpop[0], I.bpop[1], I.bpop[2], I.bpop[3], I.bpop[4],
+                                     I.bpop[5]};
+  for (int i = 0; i < kServeEntryInts; ++i) __atomic_store_n(&en[i], vals[i], __ATOMIC_RELAXED);
+  __atomic_store_n(&a->pub[0], (int)(q + 1), __ATOMIC_RELEASE);
+}
+
+bool serve_poll(ServeArena *a, int64_t q, int64_t *value, int32_t *status, double *device_ms) {
+  ServeReport *r = a->reports + (size_t)(q % a->qcap);
+  if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) != (int)(q + 1)) return false;
+  *value = __atomic_load_n(&r->value, __ATOMIC_RELAXED);
+  *status = __atomic_load_n(&r->status, __ATOMIC_RELAXED);
+  const long long t0 = __atomic_load_n(&r->t_start, __ATOMIC_RELAXED);
+  const long long t1 = __atomic_load_n(&r->t_end, __ATOMIC_RELAXED);
+  *device_ms = (double)(t1 - t0) * 1e-5;  // 100 MHz wall clock
+  return true;
+}
+
+bool serve_running(ServeArena *a) {
+  return a->launched && hipStreamQuery(a->kernel_stream) == hipErrorNotReady;
+}
+
+int64_t serve_launches(const ServeArena *a) { return a->launches; }
+
+int context_device(const mf_context *ctx) { return ctx->device; }
+
+mf_status serve_launch(ServeArena *a) {
+  HIP_TRY(hipSetDevice(a->ctx->device));
+  const ClassConfig vc = class_config(MF_ALGO_VC, 0);
+  const ClassConfig cc = class_config(MF_ALGO_TC, 0);
+  ServeArgs sa;
+  memset(&sa, 0, sizeof(sa));
+  BatchArgs &b = sa.b;
+  b.h = a->h;
+  b.e = a->e;
+  b.lists = a->lists;
+  b.N = (int)a->nv;
+  b.vcls = a->vcls;
+  b.ovf = a->words + 2;
+  b.stats = a->stats;
+  b.nclass = cc.nclass;
+  for (int c = 0; c < kNumClasses; ++c) {
+    b.lanes[c] = cc.lanes[c];
+    b.blanes[c] = vc.lanes[c];
+  }
+  b.use_vcls = 0;
+  b.ratio = 1.0f;  // cycles take about as long as the global relabel before them
+  b.max_rounds = 1000000;
+  b.max_cycles = kMaxAdaptiveCycles;
+  b.fwd_hops = kDefaultForwardHops;
+  b.stamp = a->stamp;
+  int *host_entries = nullptr, *host_pub = nullptr;
+  ServeReport *host_reports = nullptr;
+  HIP_TRY(hipHostGetDevicePointer((void **)&host_entries, a->entries, 0));
+  HIP_TRY(hipHostGetDevicePointer((void **)&host_reports, a->reports, 0));
+  HIP_TRY(hipHostGetDevicePointer((void **)&host_pub, a->pub, 0));
+  sa.entries = host_entries;
+  sa.reports = host_reports;
+  sa.published = host_pub;
+  sa.closed = host_pub + 1;
+  sa.claimed = a->words;
+  sa.pub_seen = a->words + 1;
+  sa.poll_at = (long long *)(a->words + 4);
+  sa.qcap = a->qcap;
+  sa.idle_ticks = kServeIdleTicks;
+  const int nwg = std::max(1, kServeGroupsPerCU * a->ctx->num_cu);
+  k_serve_wg<LayoutBCSR><<<nwg, kBlock, 0, a->kernel_stream>>>(LayoutBCSR{a->off, a->dc, a->rev},
+                                                               sa);
+  HIP_TRY(hipGetLastError());
+  a->launched = true;
+  a->launches++;
+  return MF_OK;
+}
+
 }  // namespace mf
+
 
 struct mf_graph {
   mf::DevGraph g;
@@ -2821,8 +3205,10 @@ mf_status mf_context_create(int device, mf_context **out) {
   ctx->num_cu = prop.multiProcessorCount;
   snprintf(ctx->name, sizeof(ctx->name), "%s", prop.name);
   snprintf(ctx->arch, sizeof(ctx->arch), "%s", prop.gcnArchName);
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete ctx;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->ev_solve[0]) != hipSuccess ||
+      hipEventCreate(&ctx->ev_solve[1]) != hipSuccess) {
+    mf_context_destroy(ctx);
     return MF_ERR_DEVICE;
   }
   *out = ctx;
@@ -2833,6 +3219,8 @@ mf_status mf_context_destroy(mf_context *ctx) {
   if (!ctx) return MF_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(ctx->device);
   for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->ev_solve)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MF_OK;
@@ -3021,6 +3409,7 @@ mf_status mf_graph_destroy(mf_context *ctx, mf_graph *mg) {
 
 mf_status mf_solve(mf_context *ctx, const mf_csr *g, mf_algo algo, mf_layout layout,
                    const mf_options *opts, mf_result *res, int32_t *flow_out) {
+
   const double t0 = now_ms();
   mf_graph *mg = nullptr;
   mf_status st = mf_graph_upload(ctx, g, layout, &mg);
