@@ -251,6 +251,18 @@ mf_status mf_graph_destroy(mf_context *ctx, mf_graph *g);
 mf_status mf_solve(mf_context *ctx, const mf_csr *g, mf_algo algo, mf_layout layout,
                    const mf_options *opts, mf_result *res, int32_t *flow_out);
 
+/* Same as mf_solve, allocating nothing on the device (no counterpart in the reference; for callers
+ * that cannot allocate: code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Every device array of the solve lives in a static 64 MiB array of the library,
+ * one per GPU, held until the process ends; its address is taken when a context is first created
+ * on the device.  BCSR only (other layouts: MF_ERR_INVALID_ARGUMENT).  Graphs whose arrays need
+ * more than 64 MiB (about 33 n + 36 m bytes, e.g. 262,144 vertices and 1.3M arcs) return
+ * MF_ERR_OVERFLOW.  cycle_graphs, record_workload and profile_kernels are ignored (off).  Calls
+ * on one GPU share its array: calls that overlap on one GPU race, so issue them from one thread
+ * (or serialise them) per device. */
+mf_status mf_solve_noalloc(mf_context *ctx, const mf_csr *g, mf_algo algo, mf_layout layout,
+                           const mf_options *opts, mf_result *res, int32_t *flow_out);
+
 /* Autotuner (no reference counterpart): trial solves of the resident graph g with candidate tile
  * widths (class_lanes, VC only), relabel cadences (relabel_time_ratio, or a fixed
  * cycles_per_relabel scaled from the cycles per round the starting options ran), forwarding

@@ -229,6 +229,8 @@ _sig("mf_graph_read_workload", C.c_int, _vp, _vp, C.POINTER(C.c_uint64), C.POINT
 _sig("mf_graph_destroy", C.c_int, _vp, _vp)
 _sig("mf_solve", C.c_int, _vp, C.POINTER(_Csr), C.c_int, C.c_int, C.POINTER(_Options),
      C.POINTER(_Result), _i32p)
+_sig("mf_solve_noalloc", C.c_int, _vp, C.POINTER(_Csr), C.c_int, C.c_int, C.POINTER(_Options),
+     C.POINTER(_Result), _i32p)
 _sig("mf_rcsr_push_relabel", C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
      _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p,
      C.POINTER(_Options), C.POINTER(_Result))
@@ -263,7 +265,7 @@ EXPORTED = [
     "mf_server_stats_get", "mf_server_destroy", "mf_footprint_bytes", "mf_flow_return_excess",
     "mf_autotune", "mf_batch_autotune", "mf_graph_read_layout", "mf_graph_read_workload",
     "mf_load_konect_bipartite", "mf_super_terminals", "mf_bfs_terminal_pairs",
-    "mf_server_request_times", "mf_server_latency",
+    "mf_server_request_times", "mf_server_latency", "mf_solve_noalloc",
 ]
 
 
@@ -562,6 +564,18 @@ class Context:
         fl = np.zeros(max(g.m, 1), dtype=np.int32) if flow else None
         _check(_lib.mf_solve(self._p, C.byref(c), algo, layout, C.byref(o), C.byref(r),
                              _ptr(fl) if flow else None), "solve")
+        res = Result._from(r)
+        return (res, fl[: g.m]) if flow else res
+
+    def solve_noalloc(self, g: Graph, algo: int = ALGO_VC, flow: bool = False, **opts):
+        """mf_solve_noalloc: BCSR solve whose device arrays live in the library's static scratch
+        array (nothing is allocated on the device)."""
+        c = g.csr()
+        o = options(**opts)
+        r = _Result()
+        fl = np.zeros(max(g.m, 1), dtype=np.int32) if flow else None
+        _check(_lib.mf_solve_noalloc(self._p, C.byref(c), algo, LAYOUT_BCSR, C.byref(o),
+                                     C.byref(r), _ptr(fl) if flow else None), "solve_noalloc")
         res = Result._from(r)
         return (res, fl[: g.m]) if flow else res
 

@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <vector>
 
 #include "build.h"
@@ -1448,6 +1449,10 @@ __global__ __launch_bounds__(kBlock) void k_preflow(L g, int *e, const int *src,
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
+__global__ void k_fill_int(int *p, int v, int count) {
+  for (int i = (int)threadIdx.x; i < count; i += (int)blockDim.x) p[i] = v;
+}
+
 __global__ void k_gather_values(const int *e, const int *snk, int k, long long *out) {
   const int stride = gridDim.x * blockDim.x;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride) out[i] = e[snk[i]];
@@ -2120,7 +2125,9 @@ static mf_status set_vertex_classes(DevGraph &G, const std::vector<int64_t> &deg
                                     hipStream_t st) {
   const int n = G.n;
   mf_status rc;
-  if ((rc = dmalloc(&G.vcls, (size_t)n)) || (rc = dmalloc(&G.vorder, (size_t)std::max(n, 1))))
+  // (arrays already placed by the caller, as in the static-scratch upload, are used as they are)
+  if ((!G.vcls && (rc = dmalloc(&G.vcls, (size_t)n))) ||
+      (!G.vorder && (rc = dmalloc(&G.vorder, (size_t)std::max(n, 1)))))
     return rc;
   const ClassConfig vc = class_config(MF_ALGO_VC, 0);
   std::vector<unsigned char> cls((size_t)n);
@@ -2138,7 +2145,7 @@ static mf_status set_vertex_classes(DevGraph &G, const std::vector<int64_t> &deg
   // giant-list state, one entry per possible list position: no minimum, no parts, nothing
   // reserved (each giant's last part restores this)
   const size_t giants = (size_t)std::max<int64_t>(G.vc_pop[kNumClasses - 1], 1);
-  if ((rc = dmalloc(&G.giant, giants * kGiantWords))) return rc;
+  if (!G.giant && (rc = dmalloc(&G.giant, giants * kGiantWords))) return rc;
   std::vector<unsigned long long> gs0(giants * kGiantWords, 0ull);
   for (size_t q = 0; q < giants; ++q) gs0[q * kGiantWords] = ~0ull;
   HIP_TRY(hipMemcpy(G.giant, gs0.data(), sizeof(unsigned long long) * gs0.size(),
@@ -3182,6 +3189,130 @@ mf_status serve_launch(ServeArena *a) {
 
 }  // namespace mf
 
+// ---- solving without allocating: device memory from a static per-device array -------------------
+//
+// mf_solve_noalloc keeps every device array of a solve (layout, vertex state, worklists, control
+// words) in `scratch`, a static device array of kScratchBytes that every GPU the library runs on
+// holds until the process ends.  Host code never passes `scratch` itself to a kernel or a copy:
+// its device address comes from hipGetSymbolAddress, taken once per device when a context is
+// created on it, and kept in g_scratch_ptr.
+
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace mf {
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kScratchDevices];  // device address of `scratch`, per device
+
+static mf_status static_scratch_setup(int device) {
+  if (device < 0 || device >= kScratchDevices) return MF_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return MF_OK;
+  void *p = nullptr;
+  HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));  // the context's device is current
+  if (!p) return MF_ERR_DEVICE;
+  g_scratch_ptr[device] = p;
+  return MF_OK;
+}
+
+// Places the BCSR arrays of an (n, m) graph and its solver state in the scratch array, every
+// region aligned to 256 bytes (base null: only sizes).  Returns the bytes used, or -1 when they
+// exceed kScratchBytes.
+template <class T>
+static void carve(char *base, size_t &at, bool &fits, T **p, size_t count) {
+  at = (at + 255) & ~(size_t)255;
+  const size_t bytes = sizeof(T) * std::max<size_t>(count, 1);
+  if (at + bytes > kScratchBytes) fits = false;
+  *p = (fits && base) ? reinterpret_cast<T *>(base + at) : nullptr;
+  at += bytes;
+}
+
+static int64_t carve_static(char *base, int32_t n, int64_t m, int64_t giants, DevGraph &G) {
+  size_t at = 0;
+  bool fits = true;
+  const size_t S = (size_t)(2 * m);
+  carve(base, at, fits, &G.off, (size_t)n + 1);
+  carve(base, at, fits, &G.dc, S);
+  carve(base, at, fits, &G.rev, S);
+  carve(base, at, fits, &G.cap0, S);
+  carve(base, at, fits, &G.fslot, (size_t)m);
+  carve(base, at, fits, &G.h, (size_t)n);
+  carve(base, at, fits, &G.e, (size_t)n);
+  carve(base, at, fits, &G.stamp, (size_t)n);
+  carve(base, at, fits, &G.lists, (size_t)4 * n);
+  carve(base, at, fits, &G.ctl, (size_t)CTL_SIZE);
+  carve(base, at, fits, &G.stats, (size_t)kStatSlot * kStatSpread);
+  carve(base, at, fits, &G.vcls, (size_t)n);
+  carve(base, at, fits, &G.vorder, (size_t)n);
+  carve(base, at, fits, &G.giant, (size_t)std::max<int64_t>(giants, 1) * kGiantWords);
+  carve(base, at, fits, &G.src, 1);
+  carve(base, at, fits, &G.snk, 1);
+  carve(base, at, fits, &G.values, 1);
+  carve(base, at, fits, &G.cycle_base, 1);
+  carve(base, at, fits, &G.cls_tab, (size_t)4 * kNumClasses);
+  return fits ? (int64_t)at : -1;
+}
+
+// Single-instance BCSR upload into the scratch array: the layout is built on the host (O(m)) and
+// copied; nothing is allocated on the device.
+static mf_status upload_static(mf_context *ctx, const mf_csr *c, DevGraph &G) {
+  SlotLayout L;
+  mf_status rc = build_slot_layout(c, L);
+  if (rc != MF_OK) return rc;
+  const int32_t n = c->n;
+  std::vector<int64_t> degree((size_t)n);
+  int64_t giants = 0;
+  const ClassConfig vc = class_config(MF_ALGO_VC, 0);
+  for (int32_t v = 0; v < n; ++v) {
+    degree[(size_t)v] = L.off[(size_t)v + 1] - L.off[(size_t)v];
+    if (vc.lanes[class_of_degree(vc, degree[(size_t)v])] == kSplitLanes) ++giants;
+  }
+  if (carve_static(nullptr, n, c->m, giants, G) < 0) return MF_ERR_OVERFLOW;
+  char *base = static_cast<char *>(g_scratch_ptr[ctx->device]);
+  if (!base) return MF_ERR_DEVICE;
+  carve_static(base, n, c->m, giants, G);
+  G.layout = MF_LAYOUT_BCSR;
+  G.n = n;
+  G.m = c->m;
+  G.S = L.slots;
+  G.k = 1;
+  G.h_off.assign(c->offsets, c->offsets + n + 1);
+  G.h_dst.assign(c->dst, c->dst + c->m);
+  G.h_cap.resize((size_t)c->m);
+  for (int64_t i = 0; i < c->m; ++i) G.h_cap[(size_t)i] = c->cap ? c->cap[i] : 1;
+  G.h_src.assign(1, c->s);
+  G.h_snk.assign(1, c->t);
+  G.h_fslot = L.fslot;
+  int64_t sc = 0;
+  for (int32_t a = c->offsets[c->s]; a < c->offsets[c->s + 1]; ++a)
+    if (c->dst[a] != c->s) sc += G.h_cap[(size_t)a];
+  if (sc > INT32_MAX) return MF_ERR_OVERFLOW;  // every excess must fit int32
+  std::vector<int2> dc((size_t)G.S);
+  for (int64_t j = 0; j < G.S; ++j) dc[(size_t)j] = make_int2(L.dst[(size_t)j], L.cap0[(size_t)j]);
+  hipStream_t st = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(G.off, L.off.data(), sizeof(int) * ((size_t)n + 1),
+                         hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(G.dc, dc.data(), sizeof(int2) * (size_t)G.S, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(G.rev, L.rev.data(), sizeof(int) * (size_t)G.S, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(G.cap0, L.cap0.data(), sizeof(int) * (size_t)G.S, hipMemcpyHostToDevice,
+                         st));
+  HIP_TRY(hipMemcpyAsync(G.fslot, L.fslot.data(), sizeof(int) * (size_t)c->m,
+                         hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(G.src, &c->s, sizeof(int), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(G.snk, &c->t, sizeof(int), hipMemcpyHostToDevice, st));
+  // control words start at zero (the previous call on this device left its own): a fill kernel
+  k_fill_int<<<1, 256, 0, st>>>(G.ctl, 0, CTL_SIZE);
+  HIP_TRY(hipGetLastError());
+  if ((rc = set_vertex_classes(G, degree, st)) != MF_OK) return rc;
+  G.max_src_deg = degree[(size_t)c->s];
+  HIP_TRY(hipStreamSynchronize(st));
+  G.fresh = true;  // the residuals were written as the capacities
+  return MF_OK;
+}
+
+}  // namespace mf
 
 struct mf_graph {
   mf::DevGraph g;
@@ -3210,6 +3341,11 @@ mf_status mf_context_create(int device, mf_context **out) {
       hipEventCreate(&ctx->ev_solve[1]) != hipSuccess) {
     mf_context_destroy(ctx);
     return MF_ERR_DEVICE;
+  }
+  const mf_status sc = static_scratch_setup(device);
+  if (sc != MF_OK) {
+    mf_context_destroy(ctx);
+    return sc;
   }
   *out = ctx;
   return MF_OK;
@@ -3405,6 +3541,32 @@ mf_status mf_graph_destroy(mf_context *ctx, mf_graph *mg) {
   free_dev(&mg->g);
   delete mg;
   return MF_OK;
+}
+
+mf_status mf_solve_noalloc(mf_context *ctx, const mf_csr *g, mf_algo algo, mf_layout layout,
+                           const mf_options *opts, mf_result *res, int32_t *flow_out) {
+  if (!ctx || (algo != MF_ALGO_TC && algo != MF_ALGO_VC) || layout != MF_LAYOUT_BCSR)
+    return MF_ERR_INVALID_ARGUMENT;
+  mf_status st = validate_csr(g);
+  if (st != MF_OK) return st;
+  mf_options o;
+  mf_options_default(&o);
+  if (opts) o = *opts;
+  if ((st = check_options(o)) != MF_OK) return st;
+  // the options whose launches need memory of their own are off: captured cycle graphs,
+  // per-wave workload records and sampled launch timing
+  o.cycle_graphs = 0;
+  o.record_workload = 0;
+  o.profile_kernels = 0;
+  HIP_TRY(hipSetDevice(ctx->device));
+  const double t0 = now_ms();
+  mf_graph view;  // every device pointer of view.g points into the scratch array: none is freed
+  st = upload_static(ctx, g, view.g);
+  const double t1 = now_ms();
+  if (st == MF_OK) st = solve_graph(ctx, view.g, algo, o, res);
+  if (res) res->upload_ms = t1 - t0;
+  if (st == MF_OK && flow_out) st = mf_graph_read_flow(ctx, &view, flow_out, 1);
+  return st;
 }
 
 mf_status mf_solve(mf_context *ctx, const mf_csr *g, mf_algo algo, mf_layout layout,
