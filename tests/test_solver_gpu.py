@@ -456,12 +456,19 @@ def test_push_forwarding(mf, orc, ctx, golden, name, hops):
 
 @pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("name", ["snap_8.txt", "m50.max", "rlg16_s3", "rmat12_s3",
-                                  "bip_500_700_s5", "multigraph"])
+                                  "bip_500_700_s5", "multigraph", "hubs_with_loops"])
 def test_device_layout_matches_host_builder(mf, ctx, golden, name, layout):
     # the residual layout built on the device is array-for-array the host builder's
     if name == "multigraph":  # parallel and antiparallel arcs, self-loops, isolated vertices
         rng = np.random.default_rng(3)
         g = random_graph(mf, rng, 300, 2500, cap_max=50)
+    elif name == "hubs_with_loops":  # hubs in both directions, self-loops placed between out-arcs
+        rng = np.random.default_rng(8)
+        n = 3000
+        u = np.concatenate([np.zeros(n, int), np.arange(n), [0, 0, 7, 7, 7], rng.integers(0, n, 6000)])
+        v = np.concatenate([np.arange(n), np.zeros(n, int), [0, 0, 7, 7, 7], rng.integers(0, n, 6000)])
+        order = rng.permutation(len(u))  # self-loops land anywhere among their tail's arcs
+        g = mf.Graph.from_arcs(n, u[order], v[order], rng.integers(1, 9, len(u))[order], 1, n - 1)
     elif name.endswith((".max", ".txt")):
         g, _ = fixture_graph(mf, golden, name)
     else:

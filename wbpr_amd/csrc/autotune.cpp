@@ -276,14 +276,12 @@ mf_status mf_batch_autotune(mf_context *ctx, const mf_csr *inst, int32_t k, int3
   if (base) o = *base;
   o.profile_kernels = 0;
   // the sample is packed and uploaded once; every trial solves it from the capacities again
-  std::vector<int32_t> off, dst, cap, src, snk, vbase;
-  mf_csr packed;
-  mf_status pst = mf::pack_instances(sub.data(), (int32_t)sub.size(), off, dst, cap, src, snk,
-                                     vbase, packed);
+  mf::PackedBatch packed;
+  mf_status pst = mf::pack_instances(ctx, sub.data(), (int32_t)sub.size(), packed);
   if (pst != MF_OK) return pst;
   mf::BatchResident *resident = nullptr;
-  pst = mf::batch_upload(ctx, &packed, src.data(), snk.data(), vbase.data(), (int32_t)sub.size(),
-                         o, &resident);
+  pst = mf::batch_upload(ctx, &packed.view, packed.src.data(), packed.snk.data(),
+                         packed.vbase.data(), (int32_t)sub.size(), o, &resident);
   if (pst != MF_OK) return pst;
   struct Release {
     mf::BatchResident *r;

@@ -2,7 +2,9 @@
 // vertex offsets.  By default a persistent grid of workgroups pulls whole instances from a
 // device queue (one workgroup solves one instance at a time); batch_mode 1 instead runs the
 // grid-wide kernels over all instances at once.
+#include <algorithm>
 #include <climits>
+#include <cstring>
 #include <vector>
 
 #include "mf_internal.h"
@@ -10,48 +12,66 @@
 
 namespace mf {
 
-// Concatenates instances: vertex v of instance i becomes vbase[i] + v.
-mf_status pack_instances(const mf_csr *inst, int32_t k, std::vector<int32_t> &off,
-                         std::vector<int32_t> &dst, std::vector<int32_t> &cap,
-                         std::vector<int32_t> &src, std::vector<int32_t> &snk,
-                         std::vector<int32_t> &vbase, mf_csr &packed) {
+// Concatenates instances: vertex v of instance i becomes vbase[i] + v, arc a of instance i arc
+// abase[i] + a.  The arrays go into the context's pinned staging buffer when it can provide one
+// (so the upload is one DMA per array), else into out.own; instances are copied in parallel.
+mf_status pack_instances(mf_context *ctx, const mf_csr *inst, int32_t k, PackedBatch &out) {
   int64_t N = 0, M = 0;
+  std::vector<int64_t> abase((size_t)k + 1);
+  out.vbase.resize((size_t)k + 1);
+  out.src.resize((size_t)k);
+  out.snk.resize((size_t)k);
+  std::vector<mf_status> valid((size_t)k, MF_OK);
+  parallel_for(
+      k,
+      [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) valid[(size_t)i] = validate_csr(&inst[i]);
+      },
+      1);
   for (int32_t i = 0; i < k; ++i) {
-    mf_status st = validate_csr(&inst[i]);
-    if (st != MF_OK) return st;
+    if (valid[(size_t)i] != MF_OK) return valid[(size_t)i];
+    if (N + inst[i].n + 2 >= INT32_MAX || 2 * (M + inst[i].m) > INT32_MAX) return MF_ERR_OVERFLOW;
+    out.vbase[(size_t)i] = (int32_t)N;
+    abase[(size_t)i] = M;
+    out.src[(size_t)i] = (int32_t)(N + inst[i].s);
+    out.snk[(size_t)i] = (int32_t)(N + inst[i].t);
     N += inst[i].n;
     M += inst[i].m;
   }
-  if (N + 2 >= INT32_MAX || 2 * M > INT32_MAX) return MF_ERR_OVERFLOW;
-  off.resize((size_t)N + 1);
-  dst.resize((size_t)M);
-  cap.resize((size_t)M);
-  src.resize((size_t)k);
-  snk.resize((size_t)k);
-  vbase.resize((size_t)k + 1);
-  int64_t vb = 0, ab = 0;
-  off[0] = 0;
-  for (int32_t i = 0; i < k; ++i) {
-    const mf_csr &g = inst[i];
-    for (int32_t v = 0; v < g.n; ++v) off[(size_t)(vb + v + 1)] = (int32_t)(ab + g.offsets[v + 1]);
-    for (int64_t a = 0; a < g.m; ++a) {
-      dst[(size_t)(ab + a)] = (int32_t)(vb + g.dst[a]);
-      cap[(size_t)(ab + a)] = g.cap ? g.cap[a] : 1;
-    }
-    vbase[(size_t)i] = (int32_t)vb;
-    src[(size_t)i] = (int32_t)(vb + g.s);
-    snk[(size_t)i] = (int32_t)(vb + g.t);
-    vb += g.n;
-    ab += g.m;
+  out.vbase[(size_t)k] = (int32_t)N;
+  abase[(size_t)k] = M;
+  const size_t words = (size_t)N + 1 + 2 * (size_t)M;
+  int32_t *buf = ctx ? static_cast<int32_t *>(context_staging(ctx, sizeof(int32_t) * words)) : nullptr;
+  if (!buf) {
+    out.own.resize(words);
+    buf = out.own.data();
   }
-  vbase[(size_t)k] = (int32_t)vb;
-  packed.n = (int32_t)N;
-  packed.m = M;
-  packed.offsets = off.data();
-  packed.dst = dst.data();
-  packed.cap = cap.data();
-  packed.s = src.empty() ? 0 : src[0];
-  packed.t = snk.empty() ? 0 : snk[0];
+  out.off = buf;
+  out.dst = buf + N + 1;
+  out.cap = buf + N + 1 + M;
+  parallel_for(
+      k,
+      [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) {
+          const mf_csr &g = inst[i];
+          const int32_t vb = out.vbase[(size_t)i];
+          const int64_t ab = abase[(size_t)i];
+          for (int32_t v = 0; v < g.n; ++v) out.off[vb + v] = (int32_t)(ab + g.offsets[v]);
+          int32_t *dd = out.dst + ab, *cc = out.cap + ab;
+          for (int64_t a = 0; a < g.m; ++a) dd[a] = vb + g.dst[a];
+          if (g.cap) memcpy(cc, g.cap, sizeof(int32_t) * (size_t)g.m);
+          else std::fill(cc, cc + g.m, 1);
+        }
+      },
+      1);
+  out.off[N] = (int32_t)M;
+  out.view.n = (int32_t)N;
+  out.view.m = M;
+  out.view.offsets = out.off;
+  out.view.dst = out.dst;
+  out.view.cap = out.cap;
+  out.view.s = k ? out.src[0] : 0;
+  out.view.t = k ? out.snk[0] : 0;
   return MF_OK;
 }
 
@@ -65,11 +85,10 @@ extern "C" mf_status mf_batch_solve(mf_context *ctx, const mf_csr *inst, int32_t
   mf_options o;
   mf_options_default(&o);
   if (opts) o = *opts;
-  std::vector<int32_t> off, dst, cap, src, snk, vbase;
-  mf_csr packed;
-  mf_status st = pack_instances(inst, k, off, dst, cap, src, snk, vbase, packed);
+  PackedBatch packed;
+  mf_status st = pack_instances(ctx, inst, k, packed);
   if (st != MF_OK) return st;
   const mf_algo algo = o.batch_algo == 1 ? MF_ALGO_VC : MF_ALGO_TC;
-  return batch_solve_packed(ctx, &packed, src.data(), snk.data(), vbase.data(), k, algo,
-                            MF_LAYOUT_BCSR, o, values, total);
+  return batch_solve_packed(ctx, &packed.view, packed.src.data(), packed.snk.data(),
+                            packed.vbase.data(), k, algo, MF_LAYOUT_BCSR, o, values, total);
 }

@@ -12,6 +12,14 @@
 //                                    (+1 for a self-loop: its forward slot comes first)
 //
 // with soff[x] = off[x] + in_off[x].  The RCSR reverse CSR is the sorted in-arc list itself.
+//
+// Both terms need no search per arc.  The in-arcs of u other than self-loops have arc indices
+// outside u's out-arc range [off[u], off[u+1]), so #{in-arcs j of u : j < i} for an out-arc i of u
+// is the same count before off[u] for every out-arc (one search per vertex, k_in_before), plus,
+// only for a vertex with self-loops, those of its self-loops before i.  Likewise the out-arcs of
+// v before an in-arc i from another tail are none (i < off[v]) or all of them (i >= off[v+1]),
+// and soff[v] - in_off[v] = off[v], so the reverse slot is off[v] + that count + the arc's rank
+// in the sorted in-arc list (pos_in[i] - in_off[v] + in_off[v]).
 // Replaces the O(m) host pass (and the reference's O(sum of squared degrees) linear searches,
 // maxflow-cuda/src/graph.cpp:386-399, maxflow-bcsr/src/graph.cpp:351-362).
 #include <hip/hip_runtime.h>
@@ -71,20 +79,39 @@ __global__ void k_slot_offsets(const int *off, const int *in_off, int n, int *so
     soff[v] = off[v] + in_off[v];
 }
 
+// Per vertex u: the number of in-arcs of u whose arc index is below off[u], negated minus one
+// when u has a self-loop (whose in-arc indices fall inside u's own out-arc range).
+__global__ void k_in_before(const int *off, const int *in_off, const int *in_list, int n,
+                            int *before) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t uu = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; uu < n; uu += stride) {
+    const int u = (int)uu;
+    const int lo = in_off[u], len = in_off[u + 1] - lo;
+    const int cb = (int)lower_bound(in_list + lo, len, off[u]);
+    const bool self = cb < len && in_list[lo + cb] < off[u + 1];
+    before[u] = self ? -cb - 1 : cb;
+  }
+}
+
 // Slot pair of every arc; layout 1 = BCSR ({dst, residual} + rev), 2 = BCSR2 (dst + {residual, rev}).
 __global__ void k_place_slots(const int *off, const int *dst, const int *cap, const int *tail,
                               const int *in_off, const int *in_list, const int *pos_in,
-                              const int *soff, int64_t m, int layout, int2 *dc, int *rev,
+                              const int *in_before, int64_t m, int layout, int2 *dc, int *rev,
                               int *dst2, int2 *cr, int *cap0, int *fslot) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t ii = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; ii < m; ii += stride) {
     const int i = (int)ii;
     const int u = tail[i], v = dst[i], c = cap ? cap[i] : 1;
-    const int lo = in_off[u];
-    const int a = soff[u] + (i - off[u]) + (int)lower_bound(in_list + lo, in_off[u + 1] - lo, i);
-    const int outdeg_v = off[v + 1] - off[v];
-    const int before = min(max(i - off[v], 0), outdeg_v);
-    const int b = soff[v] + before + (pos_in[i] - in_off[v]) + (u == v ? 1 : 0);
+    const int cb = in_before[u];
+    int earlier = cb;  // in-arcs of u with an index below i
+    if (cb < 0) {      // u has self-loops: count its in-arcs below i by search (rare)
+      const int lo = in_off[u];
+      earlier = (int)lower_bound(in_list + lo, in_off[u + 1] - lo, i);
+    }
+    const int a = in_off[u] + i + earlier;
+    const int ov = off[v], ov1 = off[v + 1];
+    const int before = u == v ? i - ov : (i >= ov1 ? ov1 - ov : 0);
+    const int b = ov + before + pos_in[i] + (u == v ? 1 : 0);
     if (layout == 1) {
       dc[a] = make_int2(v, c);
       dc[b] = make_int2(u, 0);
@@ -131,18 +158,19 @@ mf_status build_layout_device(hipStream_t st, int32_t n, int64_t m, const int *o
                               const int *dst, const int *cap, mf_layout layout,
                               const DeviceLayout &out, int *in_off_out) {
   int *tail = nullptr, *iota = nullptr, *heads = nullptr, *in_list = nullptr, *pos_in = nullptr;
-  int *in_off = nullptr;
+  int *in_off = nullptr, *in_before = nullptr;
   void *tmp = nullptr;
   mf_status result = MF_ERR_DEVICE;
   auto cleanup = [&]() {
-    void *ptrs[] = {tail, iota, heads, in_list, pos_in, in_off, tmp};
+    void *ptrs[] = {tail, iota, heads, in_list, pos_in, in_off, in_before, tmp};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
   };
   do {
     if (dalloc(&tail, m) != hipSuccess || dalloc(&iota, m) != hipSuccess ||
         dalloc(&heads, m) != hipSuccess || dalloc(&in_list, m) != hipSuccess ||
-        dalloc(&pos_in, m) != hipSuccess || dalloc(&in_off, (int64_t)n + 1) != hipSuccess) {
+        dalloc(&pos_in, m) != hipSuccess || dalloc(&in_off, (int64_t)n + 1) != hipSuccess ||
+        dalloc(&in_before, (int64_t)n) != hipSuccess) {
       result = MF_ERR_OUT_OF_MEMORY;
       break;
     }
@@ -177,8 +205,9 @@ mf_status build_layout_device(hipStream_t st, int32_t n, int64_t m, const int *o
     } else {
       k_in_position<<<ga, kThreads, 0, st>>>(in_list, m, pos_in);
       k_slot_offsets<<<gv, kThreads, 0, st>>>(off, in_off, n, out.soff);
+      k_in_before<<<gv, kThreads, 0, st>>>(off, in_off, in_list, n, in_before);
       k_place_slots<<<ga, kThreads, 0, st>>>(off, dst, cap, tail, in_off, in_list, pos_in,
-                                            out.soff, m, layout == MF_LAYOUT_BCSR ? 1 : 2, out.dc,
+                                            in_before, m, layout == MF_LAYOUT_BCSR ? 1 : 2, out.dc,
                                             out.rev, out.dst2, out.cr, out.cap0, out.fslot);
       if (hipGetLastError() != hipSuccess) break;
     }

@@ -469,9 +469,11 @@ __device__ __forceinline__ long long tile_scan(long long sv) {
   return (long long)v;
 }
 
-// Statistics accumulated per thread and flushed once per block.
+// Statistics accumulated per thread and flushed once per block.  32-bit per lane (a lane's counts
+// within one launch, or one instance of a persistent kernel, which flushes after each, stay far
+// below 2^32): 64-bit counters cost k_cycle the registers of a sixth wave per SIMD.
 struct Stats {
-  long long active, slots, pushes, relabels;
+  unsigned active, slots, pushes, relabels;
 };
 
 enum StatIdx {

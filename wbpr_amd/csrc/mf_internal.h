@@ -1,14 +1,37 @@
 // Internal declarations shared by the host-side sources of libmf.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "../../include/mf.h"
 
 namespace mf {
+
+// Runs f(begin, end) over [0, count) split into contiguous chunks, one per host thread (at most
+// 16, and none for small counts: below min_chunk items per thread the caller's thread does all).
+template <class F>
+void parallel_for(int64_t count, F f, int64_t min_chunk = 1 << 16) {
+  const int64_t hw = std::max<int64_t>(1, std::min<int64_t>(16, std::thread::hardware_concurrency()));
+  const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(hw, count / std::max<int64_t>(min_chunk, 1)));
+  if (nt <= 1) {
+    if (count > 0) f((int64_t)0, count);
+    return;
+  }
+  std::vector<std::thread> threads;
+  threads.reserve((size_t)nt - 1);
+  const int64_t step = (count + nt - 1) / nt;
+  for (int64_t t = 1; t < nt; ++t) {
+    const int64_t b = t * step, e = std::min(count, b + step);
+    if (b < e) threads.emplace_back([=, &f] { f(b, e); });
+  }
+  f((int64_t)0, std::min(count, step));
+  for (auto &th : threads) th.join();
+}
 
 // splitmix64: seeds the per-stream generators and derives per-instance seeds.
 static inline uint64_t splitmix64(uint64_t &x) {

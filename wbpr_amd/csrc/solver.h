@@ -40,11 +40,20 @@ mf_status batch_solve_packed(mf_context *ctx, const mf_csr *packed, const int32_
                              mf_layout layout, const mf_options &o, int64_t *values,
                              mf_result *res);
 
-// Concatenates instances: vertex v of instance i becomes vbase[i] + v (batch.cpp).
-mf_status pack_instances(const mf_csr *inst, int32_t k, std::vector<int32_t> &off,
-                         std::vector<int32_t> &dst, std::vector<int32_t> &cap,
-                         std::vector<int32_t> &src, std::vector<int32_t> &snk,
-                         std::vector<int32_t> &vbase, mf_csr &packed);
+// A packed batch on the host: the concatenated CSR (in the context's pinned staging buffer, or
+// in own), the global terminals and the first vertex of every instance (vbase[k] = total).
+struct PackedBatch {
+  int32_t *off = nullptr, *dst = nullptr, *cap = nullptr;
+  std::vector<int32_t> src, snk, vbase, own;
+  mf_csr view{};
+};
+// Concatenates instances: vertex v of instance i becomes vbase[i] + v (batch.cpp).  The staging
+// buffer is the context's: the packed arrays stay valid until the next call on that context that
+// stages data.
+mf_status pack_instances(mf_context *ctx, const mf_csr *inst, int32_t k, PackedBatch &out);
+// Pinned host memory of at least `bytes`, owned by the context and reused by later calls (grown
+// when needed); nullptr if it cannot be allocated.
+void *context_staging(mf_context *ctx, size_t bytes);
 
 // A packed batch uploaded once and solved again and again, with other options each time (the
 // batched autotuner's trials): every solve restarts from the capacities on the device.

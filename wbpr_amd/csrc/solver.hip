@@ -1723,6 +1723,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_wg(L g, BatchArgs b) {
       b.values[inst] = value;
       b.status[inst] = status;
     }
+    block_flush(st.active, st.slots, st.pushes, st.relabels, b.stats, ST_ACTIVE);
+    st = Stats{0, 0, 0, 0};
   }
   block_flush(st.active, st.slots, st.pushes, st.relabels, b.stats, ST_ACTIVE);
   block_flush(nv, ns, 0, 0, b.stats, ST_BFS_VERTS);
@@ -1863,6 +1865,8 @@ __global__ __launch_bounds__(kBlock) void k_serve_wg(L g, ServeArgs a) {
       __hip_atomic_store(&r->status, status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       __hip_atomic_store(&r->seq, q + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
+    block_flush(st.active, st.slots, st.pushes, st.relabels, a.b.stats, ST_ACTIVE);
+    st = Stats{0, 0, 0, 0};
     idle_from = (long long)wall_clock64();
   }
   block_flush(st.active, st.slots, st.pushes, st.relabels, a.b.stats, ST_ACTIVE);
@@ -2018,6 +2022,8 @@ struct mf_context {
   char arch[64] = {0};
   std::vector<hipEvent_t> prof_ev;
   hipEvent_t ev_solve[2] = {nullptr, nullptr};  // brackets of the solve loop's phases
+  void *staging = nullptr;  // pinned host buffer for packed batches (grown on demand)
+  size_t staging_bytes = 0;
 };
 
 namespace mf {
@@ -2070,8 +2076,9 @@ struct DevGraph {
   int64_t max_src_deg = 0;         // largest residual degree of a source
   int64_t giant_parts = 0;         // parts of all giants (kPartSlots slots each)
   unsigned long long *giant = nullptr;  // per giant-list position: state of the parts
-  // host copies for result read-back
+  // host copies for result read-back (single solves; batched uploads keep none)
   std::vector<int32_t> h_off, h_dst, h_cap, h_fslot, h_src, h_snk;
+  std::vector<int32_t> degree;  // residual slots of every vertex (out + in arcs)
   bool solved = false;
   bool fresh = false;  // residuals still equal the capacities (built, not solved since)
 
@@ -2134,13 +2141,36 @@ static mf_status set_vertex_classes(DevGraph &G, const std::vector<int64_t> &deg
   for (int q = 0; q < kNumClasses; ++q) G.vc_pop[q] = G.bu_pop[q] = 0;
   for (int q = 0; q < kNumClasses; ++q) G.class_slots[q] = 0;
   G.giant_parts = 0;
-  for (int32_t v = 0; v < n; ++v) {
-    const int q = class_of_degree(vc, degree[(size_t)v]);
-    cls[(size_t)v] = (unsigned char)q;
-    G.vc_pop[q]++;
-    G.class_slots[q] += degree[(size_t)v];
-    if (degree[(size_t)v] > 0) G.bu_pop[q]++;
-    if (vc.lanes[q] == kSplitLanes) G.giant_parts += (degree[(size_t)v] + kPartSlots - 1) / kPartSlots;
+  // classes in parallel over contiguous vertex chunks; per chunk the counts of nonzero-degree
+  // vertices by class, so the chunks fill their parts of the class-ordered list in vertex order
+  constexpr int64_t kChunk = 1 << 20;
+  const int64_t nchunk = std::max<int64_t>(1, (n + kChunk - 1) / kChunk);
+  std::vector<std::array<int64_t, 4 * kNumClasses + 1>> part((size_t)nchunk);
+  parallel_for(
+      nchunk,
+      [&](int64_t b, int64_t e) {
+        for (int64_t ch = b; ch < e; ++ch) {
+          auto &pc = part[(size_t)ch];
+          pc.fill(0);
+          for (int64_t v = ch * kChunk; v < std::min<int64_t>(n, (ch + 1) * kChunk); ++v) {
+            const int64_t d = degree[(size_t)v];
+            const int q = class_of_degree(vc, d);
+            cls[(size_t)v] = (unsigned char)q;
+            pc[q]++;                                      // vertices
+            pc[kNumClasses + q] += d;                     // slots
+            if (d > 0) pc[2 * kNumClasses + q]++;         // nonzero degree
+            if (vc.lanes[q] == kSplitLanes) pc[4 * kNumClasses] += (d + kPartSlots - 1) / kPartSlots;
+          }
+        }
+      },
+      1);
+  for (const auto &pc : part) {
+    for (int q = 0; q < kNumClasses; ++q) {
+      G.vc_pop[q] += pc[q];
+      G.class_slots[q] += pc[kNumClasses + q];
+      G.bu_pop[q] += pc[2 * kNumClasses + q];
+    }
+    G.giant_parts += pc[4 * kNumClasses];
   }
   // giant-list state, one entry per possible list position: no minimum, no parts, nothing
   // reserved (each giant's last part restores this)
@@ -2158,8 +2188,22 @@ static mf_status set_vertex_classes(DevGraph &G, const std::vector<int64_t> &deg
     G.nonzero += G.bu_pop[q];
   }
   std::vector<int> order((size_t)G.nonzero);
-  for (int32_t v = 0; v < n; ++v)
-    if (degree[(size_t)v] > 0) order[(size_t)fill[cls[(size_t)v]]++] = v;
+  for (auto &pc : part)  // each chunk's first position per class (exclusive prefix over chunks)
+    for (int q = 0; q < kNumClasses; ++q) {
+      pc[3 * kNumClasses + q] = fill[q];
+      fill[q] += (int)pc[2 * kNumClasses + q];
+    }
+  parallel_for(
+      nchunk,
+      [&](int64_t b, int64_t e) {
+        for (int64_t ch = b; ch < e; ++ch) {
+          int64_t at[kNumClasses];
+          for (int q = 0; q < kNumClasses; ++q) at[q] = part[(size_t)ch][3 * kNumClasses + q];
+          for (int64_t v = ch * kChunk; v < std::min<int64_t>(n, (ch + 1) * kChunk); ++v)
+            if (degree[(size_t)v] > 0) order[(size_t)at[cls[(size_t)v]]++] = (int)v;
+        }
+      },
+      1);
   G.vo.items = G.vorder;
   HIP_TRY(hipMemcpyAsync(G.vcls, cls.data(), (size_t)n, hipMemcpyHostToDevice, st));
   if (!order.empty())
@@ -2170,23 +2214,32 @@ static mf_status set_vertex_classes(DevGraph &G, const std::vector<int64_t> &deg
 }
 
 // Uploads a (packed) graph: c is the union CSR, src/snk the k terminal pairs.
+// keep_host: keep host copies of the CSR for flow read-back (single solves); batched uploads,
+// whose flows are never read, skip them.
 static mf_status upload(mf_context *ctx, const mf_csr *c, const int32_t *src, const int32_t *snk,
-                        int32_t k, mf_layout layout, DevGraph &G) {
+                        int32_t k, mf_layout layout, DevGraph &G, bool keep_host = true) {
   G.layout = layout;
   G.n = c->n;
   G.m = c->m;
   G.k = k;
-  G.h_off.assign(c->offsets, c->offsets + c->n + 1);
-  G.h_dst.assign(c->dst, c->dst + c->m);
-  G.h_cap.resize((size_t)c->m);
-  for (int64_t i = 0; i < c->m; ++i) G.h_cap[(size_t)i] = c->cap ? c->cap[i] : 1;
+  std::vector<int32_t> unit_cap;
+  const int32_t *cap = c->cap;
+  if (!cap) {  // unit capacities (SNAP inputs)
+    unit_cap.assign((size_t)c->m, 1);
+    cap = unit_cap.data();
+  }
+  if (keep_host) {
+    G.h_off.assign(c->offsets, c->offsets + c->n + 1);
+    G.h_dst.assign(c->dst, c->dst + c->m);
+    G.h_cap.assign(cap, cap + c->m);
+  }
   G.h_src.assign(src, src + k);
   G.h_snk.assign(snk, snk + k);
   if ((int64_t)c->n + 1 >= INT32_MAX) return MF_ERR_OVERFLOW;
   for (int32_t i = 0; i < k; ++i) {
     int64_t sc = 0;
     for (int32_t a = c->offsets[src[i]]; a < c->offsets[src[i] + 1]; ++a)
-      if (c->dst[a] != src[i]) sc += G.h_cap[(size_t)a];
+      if (c->dst[a] != src[i]) sc += cap[a];
     if (sc > INT32_MAX) return MF_ERR_OVERFLOW;  // every excess must fit int32
   }
   hipStream_t st = ctx->stream;
@@ -2207,7 +2260,7 @@ static mf_status upload(mf_context *ctx, const mf_csr *c, const int32_t *src, co
   }
   if (hipMemcpyAsync(d_off, c->offsets, sizeof(int) * (c->n + 1), hipMemcpyHostToDevice, st) ||
       hipMemcpyAsync(d_dst, c->dst, sizeof(int) * c->m, hipMemcpyHostToDevice, st) ||
-      hipMemcpyAsync(d_cap, G.h_cap.data(), sizeof(int) * c->m, hipMemcpyHostToDevice, st)) {
+      hipMemcpyAsync(d_cap, cap, sizeof(int) * c->m, hipMemcpyHostToDevice, st)) {
     free_raw();
     return MF_ERR_DEVICE;
   }
@@ -2255,9 +2308,15 @@ static mf_status upload(mf_context *ctx, const mf_csr *c, const int32_t *src, co
   rc = build_layout_device(st, c->n, c->m, d_off, d_dst, d_cap, layout, out, in_off.data());
   free_raw();
   if (rc != MF_OK) return rc;
-  for (int32_t v = 0; v < c->n; ++v)
-    degree[(size_t)v] = (int64_t)(c->offsets[v + 1] - c->offsets[v]) +
-                        (in_off[(size_t)v + 1] - in_off[(size_t)v]);
+  parallel_for(c->n, [&](int64_t b, int64_t e) {
+    for (int64_t v = b; v < e; ++v)
+      degree[(size_t)v] = (int64_t)(c->offsets[v + 1] - c->offsets[v]) +
+                          (in_off[(size_t)v + 1] - in_off[(size_t)v]);
+  });
+  G.degree.resize((size_t)c->n);
+  parallel_for(c->n, [&](int64_t b, int64_t e) {
+    for (int64_t v = b; v < e; ++v) G.degree[(size_t)v] = (int32_t)degree[(size_t)v];
+  });
   if ((rc = dmalloc(&G.h, (size_t)c->n)) || (rc = dmalloc(&G.e, (size_t)c->n)) ||
       (rc = dmalloc(&G.stamp, (size_t)c->n)) || (rc = dmalloc(&G.cycle_base, 1)) ||
       (rc = dmalloc(&G.cls_tab, (size_t)4 * kNumClasses)) ||
@@ -2741,19 +2800,23 @@ static mf_status run_batch_wg(mf_context *ctx, DevGraph &G, const int32_t *vbase
   const int32_t k = G.k;
   const ClassConfig vc = class_config(MF_ALGO_VC, 0);
   const ClassConfig cc = class_config(algo, o.tile_lanes, o.class_lanes);
-  // slot degree and class of every vertex (the same classes upload() stored on the device)
-  std::vector<int32_t> indeg((size_t)G.n, 0);
-  for (int64_t i = 0; i < G.m; ++i) indeg[(size_t)G.h_dst[(size_t)i]]++;
+  // class populations of every instance, from the slot degrees upload() recorded (the same
+  // classes it stored on the device)
   std::vector<int32_t> nver((size_t)k), pop((size_t)k * kNumClasses, 0),
       bpop((size_t)k * kNumClasses, 0), order((size_t)k);
-  for (int32_t i = 0; i < k; ++i) {
-    nver[(size_t)i] = vbase[i + 1] - vbase[i];
-    for (int32_t v = vbase[i]; v < vbase[i + 1]; ++v) {
-      const int64_t deg = (int64_t)(G.h_off[(size_t)v + 1] - G.h_off[(size_t)v]) + indeg[(size_t)v];
-      bpop[(size_t)i * kNumClasses + class_of_degree(vc, deg)]++;
-      pop[(size_t)i * kNumClasses + (cc.nclass == 1 ? 0 : class_of_degree(cc, deg))]++;
-    }
-  }
+  parallel_for(
+      k,
+      [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) {
+          nver[(size_t)i] = vbase[i + 1] - vbase[i];
+          for (int32_t v = vbase[i]; v < vbase[i + 1]; ++v) {
+            const int64_t deg = G.degree[(size_t)v];
+            bpop[(size_t)i * kNumClasses + class_of_degree(vc, deg)]++;
+            pop[(size_t)i * kNumClasses + (cc.nclass == 1 ? 0 : class_of_degree(cc, deg))]++;
+          }
+        }
+      },
+      1);
   for (int32_t i = 0; i < k; ++i) order[(size_t)i] = i;
   std::stable_sort(order.begin(), order.end(),
                    [&](int32_t x, int32_t y) { return nver[(size_t)x] > nver[(size_t)y]; });
@@ -2879,7 +2942,7 @@ mf_status batch_solve_packed(mf_context *ctx, const mf_csr *packed, const int32_
   DevGraph G;
   const bool per_group = o.batch_mode == 0;
   const double t0 = now_ms();
-  st = upload(ctx, packed, src, snk, k, per_group ? MF_LAYOUT_BCSR : layout, G);
+  st = upload(ctx, packed, src, snk, k, per_group ? MF_LAYOUT_BCSR : layout, G, false);
   const double t1 = now_ms();
   if (st == MF_OK)
     st = per_group ? run_batch_wg(ctx, G, vbase, algo, o, res) : solve_graph(ctx, G, algo, o, res);
@@ -2911,7 +2974,7 @@ mf_status batch_upload(mf_context *ctx, const mf_csr *packed, const int32_t *src
   r->vbase.assign(vbase, vbase + (size_t)k + 1);
   r->k = k;
   r->batch_mode = o.batch_mode;
-  const mf_status st = upload(ctx, packed, src, snk, k, MF_LAYOUT_BCSR, r->G);
+  const mf_status st = upload(ctx, packed, src, snk, k, MF_LAYOUT_BCSR, r->G, false);
   if (st != MF_OK) {
     batch_release(r);
     return st;
@@ -3139,6 +3202,20 @@ int64_t serve_launches(const ServeArena *a) { return a->launches; }
 
 int context_device(const mf_context *ctx) { return ctx->device; }
 
+void *context_staging(mf_context *ctx, size_t bytes) {
+  if (ctx->staging_bytes >= bytes) return ctx->staging;
+  if (ctx->staging) (void)hipHostFree(ctx->staging);
+  ctx->staging = nullptr;
+  ctx->staging_bytes = 0;
+  if (hipSetDevice(ctx->device) != hipSuccess ||
+      hipHostMalloc(&ctx->staging, bytes, hipHostMallocDefault) != hipSuccess) {
+    ctx->staging = nullptr;
+    return nullptr;
+  }
+  ctx->staging_bytes = bytes;
+  return ctx->staging;
+}
+
 mf_status serve_launch(ServeArena *a) {
   HIP_TRY(hipSetDevice(a->ctx->device));
   const ClassConfig vc = class_config(MF_ALGO_VC, 0);
@@ -3357,6 +3434,7 @@ mf_status mf_context_destroy(mf_context *ctx) {
   for (hipEvent_t e : ctx->prof_ev) (void)hipEventDestroy(e);
   for (hipEvent_t e : ctx->ev_solve)
     if (e) (void)hipEventDestroy(e);
+  if (ctx->staging) (void)hipHostFree(ctx->staging);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MF_OK;
