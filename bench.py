@@ -500,19 +500,15 @@ def bench_copy(args):
                         "traffic": 2 * nbytes}, None)
 
 
-def bench_serving(args, mf):
-    from wbpr_amd.workloads import poisson_arrivals, serving_requests
-
-    reqs, kinds = serving_requests(mf, args.requests, args.seed)
-    arrivals = poisson_arrivals(len(reqs), args.rate, args.seed + 5)
-    ctx = mf.Context(0)
-    dev, _ = device_label(ctx)
-    srv = mf.Server(ctx, hbm_budget_bytes=args.budget_gb << 30, max_inflight_launches=args.inflight)
-    done_at = {}
+def serve_schedule(mf, ctx, reqs, arrivals, budget_bytes, inflight):
+    """Submits reqs at their arrival times (seconds from the start) to one mf.Server and polls
+    until every one is done.  Returns (values, seconds, library latency, stats)."""
+    srv = mf.Server(ctx, hbm_budget_bytes=budget_bytes, max_inflight_launches=inflight)
+    done = {}
     ids = {}
     t0 = time.perf_counter()
     nxt = 0
-    while len(done_at) < len(reqs):
+    while len(done) < len(reqs):
         now = time.perf_counter() - t0
         while nxt < len(reqs) and arrivals[nxt] <= now:
             ids[srv.submit(reqs[nxt])] = nxt
@@ -521,40 +517,73 @@ def bench_serving(args, mf):
         for rid, i in list(ids.items()):
             r = srv.result(rid)
             if r is not None:
-                done_at[i] = (time.perf_counter() - t0, r[0])
+                done[i] = r[0]
                 del ids[rid]
         if nxt < len(reqs) and not ids:
             time.sleep(max(0.0, min(0.002, arrivals[nxt] - (time.perf_counter() - t0))))
         else:
-            time.sleep(0.0005)
+            time.sleep(0.0002)
     total = time.perf_counter() - t0
     stats = srv.stats()
-    lib_p50, lib_p99, lib_sps = srv.latency()  # the library's own submit -> finish clock
+    lat = srv.latency()  # (p50 ms, p99 ms, solves/s) of finish - submit on the library's clock
     srv.close()
-    lat = np.array([done_at[i][0] - arrivals[i] for i in range(len(reqs))]) * 1e3
-    # each served value equals the same request solved alone
-    check = list(range(0, len(reqs), max(1, len(reqs) // 16)))
+    return [done[i] for i in range(len(reqs))], total, lat, stats
+
+
+def check_alone(mf, ctx, reqs, values, count=16):
+    """Each served value of a spread sample equals the same request solved alone."""
+    check = list(range(0, len(reqs), max(1, len(reqs) // count)))
     for i in check:
         alone = ctx.solve(reqs[i], mf.ALGO_VC).value
-        if alone != done_at[i][1]:
-            raise SystemExit(f"request {i}: served {done_at[i][1]} != alone {alone}")
+        if alone != values[i]:
+            raise SystemExit(f"request {i}: served {values[i]} != alone {alone}")
+    return len(check)
+
+
+def bench_serving(args, mf):
+    from wbpr_amd.workloads import poisson_arrivals, serving_requests
+
+    ctx = mf.Context(0)
+    dev, _ = device_label(ctx)
+    budget = args.budget_gb << 30
+    # the serving configuration: the three generators at 1e3..1e6 vertices, at the offered load
+    reqs, kinds = serving_requests(mf, args.requests, args.seed)
+    arrivals = poisson_arrivals(len(reqs), args.rate, args.seed + 5)
+    values, total, (p50, p99, lib_sps), stats = serve_schedule(mf, ctx, reqs, arrivals, budget,
+                                                                args.inflight)
+    checked = check_alone(mf, ctx, reqs, values)
     sps = len(reqs) / total
-    p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
-    log(f"served {len(reqs)} requests in {total:.2f} s: {sps:.2f} solves/s; library latency "
-        f"p50 {lib_p50:.1f} ms, p99 {lib_p99:.1f} ms; arrival to completion seen here p50 {p50:.1f} ms, "
-        f"p99 {p99:.1f} ms; {stats['batched']} joined the serving kernel, {stats['solo']} ran alone, "
-        f"{stats['serve_kernel_launches']} serving-kernel launches")
+    log(f"served {len(reqs)} requests in {total:.2f} s: {sps:.2f} solves/s, library latency "
+        f"p50 {p50:.1f} ms, p99 {p99:.1f} ms; {stats['batched']} joined the serving kernel, "
+        f"{stats['solo']} ran alone, {stats['serve_kernel_launches']} serving-kernel launches")
+    # a high load of small requests (each joins the running serving kernel): the requests that
+    # arrive while others are on the GPU share its launches
+    breqs, _ = serving_requests(mf, args.burst_requests, args.seed + 11, vmin=1e3, vmax=1.6e4)
+    barr = poisson_arrivals(len(breqs), args.burst_rate, args.seed + 13)
+    bvalues, btotal, (bp50, bp99, bsps), bstats = serve_schedule(mf, ctx, breqs, barr, budget,
+                                                                 args.inflight)
+    bchecked = check_alone(mf, ctx, breqs, bvalues)
+    log(f"burst: {len(breqs)} requests at {args.burst_rate}/s in {btotal:.2f} s: "
+        f"{len(breqs) / btotal:.1f} solves/s, p50 {bp50:.1f} ms, p99 {bp99:.1f} ms; "
+        f"{bstats['batched']} joined, {bstats['serve_kernel_launches']} serving-kernel launches")
     config = {"workload": f"{len(reqs)} requests, Poisson arrivals at {args.rate}/s, mix of RLG / "
                           f"Genrmf / bipartite with 1e3..1e6 vertices (log-uniform), seed {args.seed}",
-              "p50_latency_ms": round(lib_p50, 2), "p99_latency_ms": round(lib_p99, 2),
+              "p50_latency_ms": round(p50, 2), "p99_latency_ms": round(p99, 2),
               "latency_is": "mf_server_latency: finish - submit on the library's clock",
-              "p50_arrival_to_seen_ms": round(p50, 2), "p99_arrival_to_seen_ms": round(p99, 2),
               "solves_per_s": round(sps, 3), "library_solves_per_s": round(lib_sps, 3),
               "offered_load_per_s": args.rate,
               "inflight_launches": args.inflight, "launches": stats["launches"],
               "joined_serving_kernel": stats["batched"], "ran_alone": stats["solo"],
               "serving_kernel_launches": stats["serve_kernel_launches"],
-              "arena_peak_bytes": stats["arena_peak_bytes"], "checked_alone": len(check),
+              "arena_peak_bytes": stats["arena_peak_bytes"], "checked_alone": checked,
+              "high_load": {
+                  "workload": f"{len(breqs)} requests, Poisson arrivals at {args.burst_rate}/s, "
+                              f"same generators at 1e3..1.6e4 vertices, seed {args.seed + 11}",
+                  "p50_latency_ms": round(bp50, 2), "p99_latency_ms": round(bp99, 2),
+                  "solves_per_s": round(len(breqs) / btotal, 2),
+                  "joined_serving_kernel": bstats["batched"], "ran_alone": bstats["solo"],
+                  "serving_kernel_launches": bstats["serve_kernel_launches"],
+                  "checked_alone": bchecked},
               "device": dev}
     return line_common(args, round(sps, 3), "solves/s", 1e3 * total / len(reqs), config,
                        None, None)
@@ -596,6 +625,10 @@ def main():
     ap.add_argument("--rate", type=float, default=8.0, help="serving: arrivals per second")
     ap.add_argument("--inflight", type=int, default=4, help="serving: concurrent launches")
     ap.add_argument("--budget-gb", type=int, default=64, help="serving: HBM budget")
+    ap.add_argument("--burst-requests", type=int, default=512,
+                    help="serving: small requests of the high-load phase")
+    ap.add_argument("--burst-rate", type=float, default=2000.0,
+                    help="serving: arrivals per second of the high-load phase")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "

@@ -163,6 +163,27 @@ def test_resolve_certified_every_layout(mf, orc, ctx, golden, layout, graphs):
         dg.close()
 
 
+@pytest.mark.parametrize("layout", ["bcsr", "bcsr2"])
+def test_resolve_large_graph_restores_changed_lines(mf, orc, ctx, layout):
+    # graphs of 2^24 residual slots or more restore, before a re-solve, only the 64-byte lines of
+    # slot pairs that the last solve's pushes changed; every re-solve, with and without replayed
+    # cycle graphs, must still give a certified maximum flow of the first solve's value
+    g = mf.gen_rmat(20, 16, seed=5)
+    assert 2 * g.m >= 1 << 24
+    dg = ctx.upload(g, mf.LAYOUTS[layout])
+    try:
+        first = None
+        for rep, graphs in enumerate([0, 0, 1, 1]):
+            r = dg.solve(mf.ALGO_VC, cycle_graphs=graphs)
+            flow = dg.read_flow(return_excess=True)
+            cert, cut = orc.certify(g, flow, r.value)
+            assert cert == "ok" and cut == r.value, (layout, rep, cert, cut)
+            first = r.value if first is None else first
+            assert r.value == first, (layout, rep)
+    finally:
+        dg.close()
+
+
 @pytest.mark.parametrize("algo", ALGOS)
 def test_genrmf64_full_size(mf, orc, ctx, golden, algo):
     g, value = generated_graph(mf, golden, "genrmf64_s1")

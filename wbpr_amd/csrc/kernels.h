@@ -37,11 +37,15 @@ __device__ __forceinline__ void st_rel(int *p, int v) {
 // BCSR re-laid for the scan: slot j holds {dst, residual} in one 8-byte word so the
 // neighbour scan is a single coalesced 64-bit load per slot; the reverse slot index is a
 // separate array read only by pushes and the BFS.
+//
+// dirty (optional): one byte per 64-byte line of slot pairs (slots 8i..8i+7) that a push changed,
+// so the next solve restores only those lines to the capacities.  Null: not tracked.
 struct LayoutBCSR {
   static constexpr int kSeg = 1;
   const int *off;
   int2 *dc;
   const int *rev;
+  unsigned char *dirty;
   __device__ __forceinline__ int2 range(int u, int) const { return make_int2(off[u], off[u + 1]); }
   __device__ __forceinline__ void scan(int, int j, int &w, int &r, int &key) const {
     int2 x = dc[j];
@@ -53,8 +57,13 @@ struct LayoutBCSR {
   __device__ __forceinline__ int residual(int key) const { return dc[key].y; }
   __device__ __forceinline__ int *res_ptr(int key) const { return &dc[key].y; }
   __device__ __forceinline__ void push(int key, int d) const {
+    const int rk = rev[key];
     atomicSub(&dc[key].y, d);
-    atomicAdd(&dc[rev[key]].y, d);
+    atomicAdd(&dc[rk].y, d);
+    if (dirty) {
+      dirty[key >> 3] = 1;
+      dirty[rk >> 3] = 1;
+    }
   }
   // BFS, two stages so a lane can keep several slots in flight:
   //   bfs_slot(j)   -> neighbour w of slot j and the index of the residual of the arc w -> u
@@ -73,6 +82,7 @@ struct LayoutBCSR2 {
   const int *off;
   const int *dst;
   int2 *cr;
+  unsigned char *dirty;  // as in LayoutBCSR, over the {residual, reverse slot} pairs
   __device__ __forceinline__ int2 range(int u, int) const { return make_int2(off[u], off[u + 1]); }
   __device__ __forceinline__ void scan(int, int j, int &w, int &r, int &key) const {
     w = dst[j];
@@ -86,6 +96,10 @@ struct LayoutBCSR2 {
     const int rv = cr[key].y;
     atomicSub(&cr[key].x, d);
     atomicAdd(&cr[rv].x, d);
+    if (dirty) {
+      dirty[key >> 3] = 1;
+      dirty[rv >> 3] = 1;
+    }
   }
   __device__ __forceinline__ void bfs_slot(int, int j, int &w, int &idx) const {
     w = dst[j];
