@@ -751,6 +751,12 @@ struct BfsArgs {
   unsigned long long *stats;
   int cls_mean[kNumClasses];  // mean residual degree of each frontier class
   int wg_slots;               // the one-workgroup kernel hands over above this many frontier slots
+  // vertices a bottom-up level examined and did not reach, by class in the regions of the
+  // vertex order (table rem_cls), double-buffered by level parity; the next bottom-up level walks
+  // only these.  Counters by level % 3.  rem_items[0] null: every level walks all vertices.
+  int *rem_items[2];
+  const int *rem_cls;
+  int *rcnt[3];
 };
 
 // LDS stages of a grid-wide BFS launch (frontier and worklist appends), or none.
@@ -1037,14 +1043,22 @@ __device__ __forceinline__ bool tile_any(bool hit) {
   }
 }
 
+// Tile (T lanes) for the still unreached vertex w of class c.  Found: w joins the next frontier
+// (and the cycle worklist if it has excess); not found: w goes to the next level's list of
+// unreached vertices (rem, when kept).
 template <int T, class L>
 __device__ __forceinline__ void bu_vertex(const L &g, const BfsArgs &a, const ListSet &out,
-                                          const BfsStages &S, int w, int lane, int level,
-                                          long long &nv, long long &ns) {
+                                          const ListSet &rem, const BfsStages &S, Stage *SR,
+                                          int w, int c, int lane, int level, long long &nv,
+                                          long long &ns) {
   constexpr int B = kBfsBatch;
+  // the vertex's height, excess and worklist class, read together
+  const int hw = a.h[w];
+  const int ew = a.e[w];
+  const int aw = a.acls ? a.acls[w] : 0;
   // reached already, or a source (tile-uniform: one address).  Sources are not scanned here:
   // the top-down levels check that no source reaches a sink
-  if (a.h[w] != a.H) return;
+  if (hw != a.H) return;
   if (lane == 0) ++nv;
   bool found = false;
 #pragma unroll
@@ -1073,27 +1087,35 @@ __device__ __forceinline__ void bu_vertex(const L &g, const BfsArgs &a, const Li
   }
   const bool mine = found && lane == 0;
   if (mine) a.h[w] = level + 1;
-  bfs_append(S, a, out, w, mine, mine && a.e[w] > 0);
+  bfs_append(S, a, out, w, mine, mine && ew > 0, c, aw);
+  if (rem.items) staged_append(*SR, rem, w, !found && lane == 0, c, a.nclass);
 }
 
 template <int T, class L>
-__device__ __forceinline__ void bu_class(const L &g, const BfsArgs &a, const VertexOrder &vo,
-                                         const ListSet &out, const BfsStages &S, int c, int tile,
+__device__ __forceinline__ void bu_class(const L &g, const BfsArgs &a, const int *items, int base,
+                                         int cnt, const ListSet &out, const ListSet &rem,
+                                         const BfsStages &S, Stage *SR, int c, int tile,
                                          int ntiles, int lane, int level, long long &nv,
                                          long long &ns) {
-  const int base = pick(vo.cls_off, c), cnt = pick(vo.cls_cnt, c);
   for (int i = tile; i < cnt; i += ntiles)
-    bu_vertex<T>(g, a, out, S, vo.items[base + i], lane, level, nv, ns);
+    bu_vertex<T>(g, a, out, rem, S, SR, items[base + i], c, lane, level, nv, ns);
 }
 
+// walk_all: the level walks every nonzero-degree vertex (the first bottom-up level after a
+// top-down one, or no lists of unreached vertices are kept); otherwise those the level before
+// left unreached.
 template <class L>
 __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(L g, BfsArgs a, Dispatch d,
-                                                          VertexOrder vo, int level) {
-  if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses)
+                                                          VertexOrder vo, int level,
+                                                          int walk_all) {
+  if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses) {
     pick(a.fcnt, (level + 2) % 3)[threadIdx.x * kCtlStride] = 0;
-  __shared__ Stage sf, sa;
+    if (a.rem_items[0]) pick(a.rcnt, (level + 2) % 3)[threadIdx.x * kCtlStride] = 0;
+  }
+  __shared__ Stage sf, sa, sr;
   stage_init(sf);
   stage_init(sa);
+  stage_init(sr);
   const BfsStages S{&sf, &sa};
   const BlockSlot bs = block_slot(d);
   const int c = bs.c;
@@ -1102,11 +1124,22 @@ __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(L g, BfsArgs a, Dispat
   ListSet out = a.fr[0];  // the frontier lists differ only in their item arrays
   out.items = (level & 1) ? a.fr[0].items : a.fr[1].items;
   out.cnt = pick(a.fcnt, (level + 1) % 3);
+  const bool all = walk_all || !a.rem_items[0];
+  const int base = pick(vo.cls_off, c);
+  const int *items = all ? vo.items : ((level & 1) ? a.rem_items[1] : a.rem_items[0]);
+  const int cnt = all ? pick(vo.cls_cnt, c) : pick(a.rcnt, level % 3)[c * kCtlStride];
+  ListSet rem;
+  rem.items = a.rem_items[0] ? ((level & 1) ? a.rem_items[0] : a.rem_items[1]) : nullptr;
+  rem.cls = a.rem_cls;
+  rem.cnt = pick(a.rcnt, (level + 1) % 3);
+  rem.cstride = kCtlStride;
+  rem.ovf = a.flags + 1;
   long long nv = 0, ns = 0;
   switch (pick(d.lanes, c)) {  // same lanes per vertex as the top-down levels
 #define MF_BU_CASE(LANES, TT)                                                                  \
   case LANES:                                                                                  \
-    bu_class<TT>(g, a, vo, out, S, c, tid / TT, nthreads / TT, tid % TT, level, nv, ns);       \
+    bu_class<TT>(g, a, items, base, cnt, out, rem, S, &sr, c, tid / TT, nthreads / TT,         \
+                 tid % TT, level, nv, ns);                                                     \
     break;
     MF_BU_CASE(1, 1)
     MF_BU_CASE(8, 2)
@@ -1114,12 +1147,14 @@ __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(L g, BfsArgs a, Dispat
     MF_BU_CASE(32, 8)
     MF_BU_CASE(64, 16)
     default:
-      bu_class<64>(g, a, vo, out, S, c, tid / 64, nthreads / 64, tid % 64, level, nv, ns);
+      bu_class<64>(g, a, items, base, cnt, out, rem, S, &sr, c, tid / 64, nthreads / 64,
+                   tid % 64, level, nv, ns);
       break;
 #undef MF_BU_CASE
   }
   stage_flush(sf, out, a.nclass);
   stage_flush(sa, a.active, a.nclass_act);
+  if (rem.items) stage_flush(sr, rem, a.nclass);
   block_flush(nv, ns, 0, 0, a.stats, ST_BU_VERTS);
 }
 
@@ -1619,6 +1654,9 @@ __device__ __forceinline__ long long batch_instance(const L &g, const BatchArgs 
   ba.stats = nullptr;
   for (int c = 0; c < kNumClasses; ++c) ba.cls_mean[c] = 0;  // (used by the grid-wide BFS only)
   ba.wg_slots = INT_MAX;
+  ba.rem_items[0] = ba.rem_items[1] = nullptr;
+  ba.rem_cls = nullptr;
+  for (int q = 0; q < 3; ++q) ba.rcnt[q] = nullptr;
   CycleArgs ca;
   ca.h = b.h;
   ca.e = b.e;
@@ -2122,6 +2160,7 @@ struct DevGraph {
   bool solved = false;
   bool fresh = false;  // residuals still equal the capacities (built, not solved since)
   unsigned char *dirty = nullptr;  // BCSR/BCSR2 lines of slot pairs pushes changed (large graphs)
+  int *bu_rem = nullptr;  // two lists of unreached vertices (2 x nonzero) + their class table
 
   LayoutBCSR bcsr() const { return LayoutBCSR{off, dc, rev, dirty}; }
   LayoutBCSR2 bcsr2() const { return LayoutBCSR2{off, dst, cr, dirty}; }
@@ -2136,7 +2175,9 @@ enum {
   CTL_STATE = CTL_FLAGS + kCtlStride,                  // [0] level, [1] frontier size,
                                                        // [2..3] grid levels' marks,
                                                        // [4..5] slots explored
-  CTL_SIZE = CTL_STATE + kCtlStride
+  CTL_RCNT = CTL_STATE + kCtlStride,                   // 3 x kNumClasses unreached-list counters
+  CTL_READ = CTL_RCNT,                                 // the part the host reads back
+  CTL_SIZE = CTL_RCNT + 3 * kNumClasses * kCtlStride
 };
 
 // counter c of triple-buffer slot q of the frontier (CTL_FCNT) or worklist (CTL_CCNT) set
@@ -2150,7 +2191,8 @@ static void free_dev(DevGraph *g) {
   void *ptrs[] = {g->off, g->dc,  g->rev, g->dst,    g->cr,    g->fdc,   g->bcf,
                   g->roff, g->rdi, g->cap0, g->src,   g->snk,   g->values, g->h,
                   g->e,   g->lists, g->ctl, g->stats, g->vcls, g->vorder, g->stamp, g->fslot,
-                  g->work, g->work_cls, g->cycle_base, g->cls_tab, g->giant, g->dirty};
+                  g->work, g->work_cls, g->cycle_base, g->cls_tab, g->giant, g->dirty,
+                  g->bu_rem};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -2371,6 +2413,15 @@ static mf_status upload(mf_context *ctx, const mf_csr *c, const int32_t *src, co
   HIP_TRY(hipMemcpyAsync(G.src, src, sizeof(int) * k, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(G.snk, snk, sizeof(int) * k, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemsetAsync(G.ctl, 0, sizeof(int) * CTL_SIZE, st));
+  if (keep_host && G.nonzero >= kBottomUpMinVertices) {  // bottom-up levels' unreached lists
+    if ((rc = dmalloc(&G.bu_rem, (size_t)2 * G.nonzero + 2 * kNumClasses))) return rc;
+    int tab[2 * kNumClasses];
+    for (int c = 0; c < kNumClasses; ++c) {
+      tab[c] = G.vo.cls_off[c];
+      tab[kNumClasses + c] = G.vo.cls_cnt[c];
+    }
+    HIP_TRY(hipMemcpyAsync(G.bu_rem + 2 * G.nonzero, tab, sizeof(tab), hipMemcpyHostToDevice, st));
+  }
   if (keep_host && layout != MF_LAYOUT_RCSR && G.S >= kDirtyMinSlots) {  // single graphs only
     if ((rc = dmalloc(&G.dirty, (size_t)(G.S + 7) / 8))) return rc;
     HIP_TRY(hipMemsetAsync(G.dirty, 0, (size_t)(G.S + 7) / 8, st));
@@ -2455,6 +2506,10 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     if (bcfg.lanes[c] == kSplitLanes) ba.cls_mean[c] = std::max(ba.cls_mean[c], kWgBfsSlots + 1);
   ba.wg_slots = kWgBfsSlots;
   ba.stats = G.stats;
+  ba.rem_items[0] = G.bu_rem;
+  ba.rem_items[1] = G.bu_rem ? G.bu_rem + G.nonzero : nullptr;
+  ba.rem_cls = G.bu_rem ? G.bu_rem + 2 * G.nonzero : nullptr;
+  for (int q = 0; q < 3; ++q) ba.rcnt[q] = G.ctl + CTL_RCNT + q * kNumClasses * kCtlStride;
 
   CycleArgs ca;
   ca.h = G.h;
@@ -2602,12 +2657,12 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     for (;;) {
       k_bfs_wg<L><<<1, kWgBlock, 0, st>>>(lay, ba, bdisp, level, fmax);
       HIP_TRY(hipGetLastError());
-      HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_SIZE, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_READ, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
       level = hstate[CTL_STATE];
       int size = hstate[CTL_STATE + 1];
       if (size == 0) break;
-      bool bottom_up = false;
+      bool bottom_up = false, bu_prev = false;
       for (;;) {  // frontier too large for one workgroup: grid launches per level
         // frontier slots estimated from its class counts and the classes' mean degrees
         const double mf = frontier_slots(level);
@@ -2621,7 +2676,12 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
         bottom_up = bottom_up || bu_always;
         int next;
         if (bottom_up) {
-          k_bfs_bottom_up<L><<<bugrid, kBlock, 0, st>>>(lay, ba, budisp, G.vo, level);
+          // the first bottom-up level of a run walks every vertex; its unreached list, which the
+          // next bottom-up level walks, starts empty
+          if (!bu_prev && G.bu_rem)
+            k_fill_int<<<1, 256, 0, st>>>(ba.rcnt[(level + 1) % 3], 0, kNumClasses * kCtlStride);
+          k_bfs_bottom_up<L><<<bugrid, kBlock, 0, st>>>(lay, ba, budisp, G.vo, level, !bu_prev);
+          bu_prev = true;
           ++bfs_bu_levels;
           explored += mf;
           next = level + 1;
@@ -2640,9 +2700,10 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
           for (int q = 0; q < kGridLevelBatch; ++q)
             k_bfs_level<L><<<bgrid, kBlock, 0, st>>>(lay, ba, bdisp, level + q, ls);
           next = -1;
+          bu_prev = false;
         }
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_SIZE, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_READ, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         if (next < 0) next = std::max(hstate[CTL_STATE + 2], hstate[CTL_STATE + 3]);
         bfs_grid_levels += next - level;
@@ -2756,7 +2817,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       }
       HIP_TRY(hipGetLastError());
       done += chunk;
-      HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_SIZE, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_READ, hipMemcpyDeviceToHost, st));
       if (pending) HIP_TRY(collect(half ^ 1, pending));
       pending = timed;
       ++nchunk;
