@@ -1849,8 +1849,9 @@ struct ServeArgs {
   int *claimed;                // device memory: entries claimed so far
   int *pub_seen;               // device memory: the largest published count read so far
   long long *poll_at;          // device memory: wall_clock64 of the last host read
+  long long *last_claim;       // device memory: wall_clock64 of the last claim by any workgroup
   int qcap;
-  long long idle_ticks;        // ticks an idle workgroup waits for work before it leaves
+  long long idle_ticks;        // ticks without any claim after which idle workgroups leave
 };
 constexpr long long kHostPollTicks = 500;  // 5 us at the 100 MHz wall clock
 
@@ -1895,7 +1896,15 @@ __global__ __launch_bounds__(kBlock) void k_serve_wg(L g, ServeArgs a) {
         }
         c = old;
       }
-      if (q < 0 && ((long long)wall_clock64() - idle_from > a.idle_ticks ||
+      const long long now = (long long)wall_clock64();
+      if (q >= 0)
+        __hip_atomic_fetch_max(a.last_claim, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // the grid leaves together: an idle workgroup leaves once no workgroup has claimed an
+      // entry for idle_ticks, so requests that arrive one by one keep the whole grid, not only
+      // the workgroups that happened to be busy
+      const long long since = max(idle_from, __hip_atomic_load(a.last_claim, __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT));
+      if (q < 0 && (now - since > a.idle_ticks ||
                     __hip_atomic_load(a.closed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)))
         q = -2;
       s_q = q;
@@ -1999,6 +2008,11 @@ constexpr int64_t kBottomUpAlpha = 14;
 constexpr int64_t kBottomUpBeta = 24;
 constexpr int64_t kBottomUpMinVertices = 1 << 16;
 constexpr int kBatchGroupsPerCU = 4;         // at most this many batched workgroups per CU
+// Top-down grid levels: blocks per CU per class.  Most of a batch's levels after the first find
+// nothing to do (LevelStop), and every block of such a launch still starts and reads the
+// counters: R-MAT 22 solves take 7.63 / 7.48 / 7.45 / 7.40 ms at 8 / 4 / 2 / 1 blocks, Genrmf 64
+// is level within its spread.
+constexpr int kBfsBlocksPerCU = 2;
 
 template <class T>
 static mf_status dmalloc(T **p, size_t count) {
@@ -2040,10 +2054,10 @@ int class_of_degree(const ClassConfig &c, int64_t deg) {
   return c.nclass - 1;
 }
 
-// Blocks per class: enough tiles for the class population, at most 8 per CU; the split class
-// (giants) gets one block per part (split_units parts in all).
+// Blocks per class: enough tiles for the class population, at most per_cu per CU; the split
+// class (giants) gets one block per part (split_units parts in all).
 static Dispatch make_dispatch(const ClassConfig &c, const int64_t *pop, int num_cu,
-                              int64_t split_units) {
+                              int64_t split_units, int per_cu = 8) {
   Dispatch d;
   memset(&d, 0, sizeof(d));
   d.nclass = c.nclass;
@@ -2055,11 +2069,11 @@ static Dispatch make_dispatch(const ClassConfig &c, const int64_t *pop, int num_
     int64_t nb;
     if (c.lanes[k] <= 64) {
       const int64_t tpb = kBlock / c.lanes[k];
-      nb = std::min<int64_t>((pop[k] + tpb - 1) / tpb, (int64_t)8 * num_cu);
+      nb = std::min<int64_t>((pop[k] + tpb - 1) / tpb, (int64_t)per_cu * num_cu);
     } else if (c.lanes[k] == kSplitLanes) {
-      nb = std::min<int64_t>(std::max<int64_t>(split_units, pop[k]), (int64_t)8 * num_cu);
+      nb = std::min<int64_t>(std::max<int64_t>(split_units, pop[k]), (int64_t)per_cu * num_cu);
     } else {
-      nb = std::min<int64_t>(pop[k], (int64_t)8 * num_cu);
+      nb = std::min<int64_t>(pop[k], (int64_t)per_cu * num_cu);
     }
     b += (int)std::max<int64_t>(nb, 1);
   }
@@ -2446,7 +2460,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   const int grid = disp.blk_begin[cfg.nclass];
   // the global relabel always splits its frontier by degree class, whatever the cycle kernels do
   const ClassConfig bcfg = class_config(MF_ALGO_VC, 0);
-  const Dispatch bdisp = make_dispatch(bcfg, G.vc_pop, ctx->num_cu, G.giant_parts);
+  const Dispatch bdisp = make_dispatch(bcfg, G.vc_pop, ctx->num_cu, G.giant_parts, kBfsBlocksPerCU);
   const int bgrid = bdisp.blk_begin[bcfg.nclass];
   // bottom-up levels walk the nonzero-degree vertices in class order
   const Dispatch budisp = make_dispatch(bcfg, G.bu_pop, ctx->num_cu, 0);
@@ -3135,7 +3149,8 @@ struct ServeArena {
   int *lists = nullptr;
   int2 *dc = nullptr;
   unsigned char *vcls = nullptr;
-  int *words = nullptr;  // [0] claimed, [1] published count seen, [2] overflow, [4..5] poll time
+  int *words = nullptr;  // [0] claimed, [1] published count seen, [2] overflow, [4..5] poll time,
+                         // [6..7] last claim time
   unsigned long long *stats = nullptr;
   // host memory the GPU reads and writes uncached
   int *entries = nullptr;
@@ -3364,6 +3379,7 @@ mf_status serve_launch(ServeArena *a) {
   sa.claimed = a->words;
   sa.pub_seen = a->words + 1;
   sa.poll_at = (long long *)(a->words + 4);
+  sa.last_claim = (long long *)(a->words + 6);
   sa.qcap = a->qcap;
   sa.idle_ticks = kServeIdleTicks;
   const int nwg = std::max(1, kServeGroupsPerCU * a->ctx->num_cu);
