@@ -79,15 +79,19 @@ WORKLOADS = {
 # relabel (8.3 ms) and with the adaptive cadence (7.97 ms); over 9 solves each the two ran 8.0 and
 # 7.95 ms.  The fixed cadence is committed: it launches the same cycles whether or not they are
 # replayed from captured graphs, so the counter runs (which cannot replay graphs) measure the
-# launches the timed run makes.  Genrmf 64: 156 trials, best 98.2 ms.  Workloads not listed run
-# the library defaults.
+# launches the timed run makes; cooperative BFS levels cost R-MAT 0.16 ms (7.50 vs 7.66 ms).
+# Genrmf 64: 156 trials, best 98.2 ms; then cooperative BFS levels for mid-size frontiers with the
+# one-workgroup kernel's limit lowered to 256 vertices: 93.9 against 99.2 ms (4 solves each,
+# interleaved).  Workloads not listed run the library defaults.
 TUNED = {
     ("rmat22", "vc"): {"cycles_per_relabel": 12, "tile_lanes": 0, "bfs_block_frontier": 1024,
                        "relabel_time_ratio": 2.0, "bfs_direction": 0,
-                       "class_lanes": [4, 8, 16, 64, 256], "forward_hops": -1, "cycle_graphs": 1},
-    ("genrmf64", "vc"): {"cycles_per_relabel": 674, "tile_lanes": 0, "bfs_block_frontier": 1024,
+                       "class_lanes": [4, 8, 16, 64, 256], "forward_hops": -1, "cycle_graphs": 1,
+                       "bfs_cooperative": 0},
+    ("genrmf64", "vc"): {"cycles_per_relabel": 674, "tile_lanes": 0, "bfs_block_frontier": 256,
                          "relabel_time_ratio": 4.0, "bfs_direction": 0,
-                         "class_lanes": [4, 8, 16, 64, 256], "forward_hops": 2, "cycle_graphs": 0},
+                         "class_lanes": [4, 8, 16, 64, 256], "forward_hops": 2, "cycle_graphs": 0,
+                         "bfs_cooperative": 1},
 }
 
 BATCH_K = 4096
@@ -514,11 +518,8 @@ def serve_schedule(mf, ctx, reqs, arrivals, budget_bytes, inflight):
             ids[srv.submit(reqs[nxt])] = nxt
             nxt += 1
         srv.progress()
-        for rid, i in list(ids.items()):
-            r = srv.result(rid)
-            if r is not None:
-                done[i] = r[0]
-                del ids[rid]
+        for rid in srv.completed():
+            done[ids.pop(rid)] = srv.result(rid)[0]
         if nxt < len(reqs) and not ids:
             time.sleep(max(0.0, min(0.002, arrivals[nxt] - (time.perf_counter() - t0))))
         else:

@@ -113,7 +113,11 @@ typedef struct mf_options {
   int32_t cycle_graphs;       /* 1: issue the push-relabel cycles in chunks captured as hipGraphs
                                  (replayed, one per chunk length, captured once per solve);
                                  0: one launch per cycle */
-  int32_t reserved[2];
+  int32_t bfs_cooperative;    /* global relabel: 1 = top-down levels of mid-size frontiers run
+                                 inside one cooperative launch of a few workgroups with a grid
+                                 barrier per level (where the device supports it; for long-path
+                                 graphs); 0 = one grid launch per level (default) */
+  int32_t reserved[1];
 } mf_options;
 
 typedef struct mf_result {
@@ -355,6 +359,10 @@ mf_status mf_server_submit(mf_server *srv, const mf_csr *g, int64_t *request_id)
 mf_status mf_server_progress(mf_server *srv, int32_t wait);
 /* Result of a request: MF_ERR_NOT_READY until it has completed. */
 mf_status mf_server_result(mf_server *srv, int64_t request_id, int64_t *value, double *device_ms);
+/* Completion queue: up to cap ids of requests that completed (or failed) since the previous
+ * call, in completion order; *count receives how many.  A loop over many requests in flight
+ * asks this instead of polling every id. */
+mf_status mf_server_completed(mf_server *srv, int64_t *ids, int32_t cap, int32_t *count);
 /* Times of a completed request (MF_ERR_NOT_READY until it has completed). */
 mf_status mf_server_request_times(mf_server *srv, int64_t request_id, mf_server_times *out);
 /* Latency (finish - submit) percentiles over the completed requests (linear interpolation

@@ -92,14 +92,15 @@ def test_server_continuous_batching(mf, ctx, golden):
             if i % 5 == 4:
                 srv.progress()  # requests keep arriving while earlier ones run
         pending = dict(ids)
+        seen = []  # the completion queue hands out every id once, as it completes
         t0 = time.time()
         while pending and time.time() - t0 < 120:
             srv.progress(wait=True)
-            for rid in list(pending):
-                r = srv.result(rid)
-                if r is not None:
-                    assert r[0] == pending.pop(rid)
+            for rid in srv.completed():
+                seen.append(rid)
+                assert srv.result(rid)[0] == pending.pop(rid)
         assert not pending
+        assert sorted(seen) == sorted(rid for rid, _ in ids) and srv.completed() == []
         st = srv.stats()
         assert st["completed"] == len(inst) and st["submitted"] == len(inst)
         assert st["arena_bytes"] == 0 and 0 < st["arena_peak_bytes"] <= 64 << 20

@@ -93,7 +93,8 @@ def test_launch_geometry_stress(mf, orc, ctx, golden):
                         tile_lanes=int(rng.choice([0, 1, 8, 16, 32, 64, 256])),
                         bfs_block_frontier=int(rng.choice([1, 8, 64, 4096])),
                         bfs_direction=int(rng.choice([0, 1, 2])),
-                        forward_hops=int(rng.choice([-1, 0, 1, 32])))
+                        forward_hops=int(rng.choice([-1, 0, 1, 32])),
+                        bfs_cooperative=int(rng.choice([0, 1])))
             r = dg.solve(mf.ALGO_VC, **opts)
             assert r.value == value, opts
     finally:
@@ -306,6 +307,20 @@ def test_bfs_lds_groups_spill(mf, orc, ctx, algo, layout):
         g = layered_graph(mf, seed, narrow=narrow)
         r = solve_and_certify(mf, orc, ctx, g, algo, layout, orc.dinic_maxflow(g))
         assert r.bfs_grid_levels > 0
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("algo", ALGOS)
+def test_bfs_cooperative_levels(mf, orc, ctx, golden, algo, layout):
+    # the global relabel's mid-size frontiers run level after level inside one cooperative
+    # launch with a grid barrier per level; a low one-workgroup limit sends most of Genrmf 64's
+    # levels there, and R-MAT 16 mixes them with bottom-up levels and giant frontiers
+    g, value = generated_graph(mf, golden, "genrmf64_s1")
+    r = solve_and_certify(mf, orc, ctx, g, algo, layout, value, bfs_cooperative=1,
+                          bfs_block_frontier=64)
+    assert r.bfs_grid_levels > 0
+    g, value = generated_graph(mf, golden, "rmat16_s1")
+    solve_and_certify(mf, orc, ctx, g, algo, layout, value, bfs_cooperative=1)
 
 
 def test_rmat22_full_size(mf, orc, ctx, golden):

@@ -103,7 +103,8 @@ class _Options(C.Structure):
         ("forward_hops", C.c_int32),
         ("record_workload", C.c_int32),
         ("cycle_graphs", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("bfs_cooperative", C.c_int32),
+        ("reserved", C.c_int32 * 1),
     ]
 
     def to_dict(self) -> dict:
@@ -113,7 +114,8 @@ class _Options(C.Structure):
                 "relabel_time_ratio": round(float(self.relabel_time_ratio), 4),
                 "batch_algo": self.batch_algo, "batch_mode": self.batch_mode,
                 "bfs_direction": self.bfs_direction, "class_lanes": list(self.class_lanes),
-                "forward_hops": self.forward_hops, "cycle_graphs": self.cycle_graphs}
+                "forward_hops": self.forward_hops, "cycle_graphs": self.cycle_graphs,
+                "bfs_cooperative": self.bfs_cooperative}
 
 
 class _Result(C.Structure):
@@ -240,6 +242,7 @@ _sig("mf_server_create", C.c_int, _vp, C.c_int64, C.c_int32, C.POINTER(_vp))
 _sig("mf_server_submit", C.c_int, _vp, C.POINTER(_Csr), C.POINTER(C.c_int64))
 _sig("mf_server_progress", C.c_int, _vp, C.c_int32)
 _sig("mf_server_result", C.c_int, _vp, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_double))
+_sig("mf_server_completed", C.c_int, _vp, C.POINTER(C.c_int64), C.c_int32, C.POINTER(C.c_int32))
 _sig("mf_server_stats_get", C.c_int, _vp, C.POINTER(_ServerStats))
 _sig("mf_server_request_times", C.c_int, _vp, C.c_int64, C.POINTER(_ServerTimes))
 _sig("mf_server_latency", C.c_int, _vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -265,7 +268,7 @@ EXPORTED = [
     "mf_server_stats_get", "mf_server_destroy", "mf_footprint_bytes", "mf_flow_return_excess",
     "mf_autotune", "mf_batch_autotune", "mf_graph_read_layout", "mf_graph_read_workload",
     "mf_load_konect_bipartite", "mf_super_terminals", "mf_bfs_terminal_pairs",
-    "mf_server_request_times", "mf_server_latency", "mf_solve_noalloc",
+    "mf_server_request_times", "mf_server_latency", "mf_solve_noalloc", "mf_server_completed",
 ]
 
 
@@ -719,6 +722,17 @@ class Server:
             return None
         _check(st, "server_result")
         return v.value, ms.value
+
+    def completed(self, cap: int = 4096):
+        """Ids of the requests that completed since the previous call, in completion order."""
+        ids = (C.c_int64 * cap)()
+        k = C.c_int32()
+        out = []
+        while True:
+            _check(_lib.mf_server_completed(self._p, ids, cap, C.byref(k)), "server_completed")
+            out.extend(ids[i] for i in range(k.value))
+            if k.value < cap:
+                return out
 
     def times(self, rid: int):
         """Host times (ms since the server was created) of a completed request, or None."""

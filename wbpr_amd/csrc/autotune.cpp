@@ -14,7 +14,9 @@
 //                     moves with the run-to-run spread of the cycles; a fixed one does not (Genrmf
 //                     64: 6% less time on average and half the spread).
 //   BFS hand-over     mf_options.bfs_block_frontier: frontier size up to which all BFS levels run
-//                     inside one workgroup.
+//                     inside one workgroup.  mf_options.bfs_cooperative (single instance): the
+//                     levels of mid-size frontiers inside one cooperative launch or one launch
+//                     per level (long-path graphs gain, power-law graphs lose).
 //   forwarding        mf_options.forward_hops: vertices a tile discharges per cycle by following
 //                     its own pushes (long augmenting paths want many, churning graphs few).
 //   graph replay      mf_options.cycle_graphs (single instance): cycle chunks replayed from
@@ -53,7 +55,7 @@ const uint16_t kLanePresets[][5] = {
 };
 const float kRatios[] = {0.5f, 1.0f, 2.0f, 4.0f, 8.0f};
 const int32_t kForwardHops[] = {-1, 2, 8, 32};
-const int32_t kFrontiers[] = {1024, 4096, 16384};
+const int32_t kFrontiers[] = {256, 1024, 4096, 16384};
 const double kCadenceFactors[] = {1.0, 1.5, 2.0};
 
 float effective_ratio(const mf_options &o) {
@@ -130,6 +132,9 @@ mf_status sweep(const mf_options &cur, bool tune_lanes, bool tune_batch_algo,
     }
   }
   if (!tune_batch_algo) {  // the batched kernel runs its BFS inside each workgroup anyway
+    mf_options c = cur;
+    c.bfs_cooperative = cur.bfs_cooperative ? 0 : 1;
+    if ((st = consider(c)) != MF_OK) return st;
     const int32_t f0 = effective_frontier(cur);
     for (int32_t f : kFrontiers) {
       if (f == f0) continue;

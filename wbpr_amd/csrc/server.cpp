@@ -147,6 +147,7 @@ struct mf_server {
   std::deque<std::shared_ptr<Request>> pending;
   std::map<int64_t, std::shared_ptr<Request>> all;
   std::vector<double> latencies;  // finish - submit of completed requests, ms
+  std::deque<int64_t> completed;   // ids not yet handed out by mf_server_completed
   double first_submit = -1, last_finish = 0;
   int64_t next_id = 1;
   mf_server_stats st{};
@@ -173,6 +174,7 @@ static void finish(mf_server *srv, Request &r, State s) {
   r.state = s;
   r.t_finish = now_ms() - srv->t0;
   srv->st.completed++;
+  srv->completed.push_back(r.id);
   if (s == DONE) {
     srv->latencies.push_back(r.t_finish - r.t_submit);
     srv->last_finish = std::max(srv->last_finish, r.t_finish);
@@ -348,8 +350,12 @@ static mf_status step(mf_server *srv, std::unique_lock<std::mutex> &lk) {
   poll_batched(srv);
   admit(srv);
   publish_copied(srv);
-  // published entries that no running workgroup will take: (re)launch the serving kernel
-  if (!srv->published.empty() && !serve_running(srv->arena)) {
+  // published entries that no running workgroup will take: (re)launch the serving kernel.  Once
+  // the grid has left, every report it wrote is visible: collect them first, so an entry that
+  // was solved while this step admitted or copied is not mistaken for one left unclaimed
+  const bool running = serve_running(srv->arena);
+  if (!running) poll_batched(srv);
+  if (!srv->published.empty() && !running) {
     const mf_status s = serve_launch(srv->arena);
     if (s != MF_OK) return s;
     srv->st.launches++;
@@ -465,6 +471,18 @@ mf_status mf_server_result(mf_server *srv, int64_t request_id, int64_t *value, d
   if (r.state != DONE) return MF_ERR_NOT_READY;
   if (value) *value = r.value;
   if (device_ms) *device_ms = r.device_ms;
+  return MF_OK;
+}
+
+mf_status mf_server_completed(mf_server *srv, int64_t *ids, int32_t cap, int32_t *count) {
+  if (!srv || !count || cap < 0 || (cap > 0 && !ids)) return MF_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(srv->mu);
+  int32_t k = 0;
+  while (k < cap && !srv->completed.empty()) {
+    ids[k++] = srv->completed.front();
+    srv->completed.pop_front();
+  }
+  *count = k;
   return MF_OK;
 }
 

@@ -1016,6 +1016,115 @@ __global__ __launch_bounds__(kBlock) void k_bfs_level(L g, BfsArgs a, Dispatch d
   block_flush(nv, ns, 0, 0, a.stats, ST_BFS_VERTS);
 }
 
+// ---- cooperative top-down levels (mid-size frontiers) ----
+//
+// A frontier too large for the one-workgroup kernel and small next to the grid: kCoopBlocks
+// co-resident workgroups (a cooperative launch) run level after level, each level as
+// k_bfs_level runs it but over all classes with tiles numbered across the blocks, and a grid
+// barrier between levels: no launch and no host round trip per level.  After each barrier every
+// block reads the same complete counters and decides alike whether to go on (CoopStop).  The
+// barrier's agent-scope release and acquire make a level's list items and counters, written by
+// blocks on other XCDs, visible to the next level.  A barrier that does not complete within its
+// spin limit sets *abort and every block leaves; the host then redoes the global relabel without
+// cooperative levels.
+
+struct CoopStop {
+  int small;              // back to the one-workgroup kernel at or below this frontier size,
+  long long small_slots;  //   this many estimated slots and fitting its LDS groups
+  long long max_slots;    // back to the host (grid-wide or bottom-up levels) above this many
+};
+
+constexpr unsigned kCoopSpinLimit = 1u << 20;  // barrier polls before giving up (about a second)
+
+// Grid barrier: blocks count arrivals on *bar (zero at launch); the barrier of generation gen
+// waits for gen * gridDim.x arrivals.  False if it gave up.
+__device__ __forceinline__ bool coop_barrier(unsigned *bar, unsigned target, int *abort) {
+  __shared__ int s_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    __threadfence();  // the block's level is visible device-wide before it arrives
+    atomicAdd(bar, 1u);
+    unsigned spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+          ++spins > kCoopSpinLimit) {
+        atomicExch(abort, 1);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();  // every other block's level is visible here
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_bfs_coop(L g, BfsArgs a, Dispatch d, int level,
+                                                     CoopStop cs, unsigned *bar, int *abort) {
+  __shared__ Stage sf, sa;
+  __shared__ int s_go, s_size;
+  __shared__ long long s_slots;
+  const int gtid = (int)(blockIdx.x * kBlock + threadIdx.x);
+  const int gthreads = (int)(gridDim.x * kBlock);
+  long long nv = 0, ns = 0;
+  for (unsigned gen = 1;; ++gen) {
+    ListSet in = a.fr[0];
+    in.items = (level & 1) ? a.fr[1].items : a.fr[0].items;
+    in.cnt = pick(a.fcnt, level % 3);
+    if (threadIdx.x == 0) {
+      int size = 0, ga = 0, gb = 0, gc = 0;
+      long long slots = 0;
+      for (int c = 0; c < d.nclass; ++c) {
+        const int k = list_count(in, c);
+        size += k;
+        slots += (long long)k * pick(a.cls_mean, c);
+        const int q = wg_group(c);
+        ga += q == 0 ? k : 0;
+        gb += q == 1 ? k : 0;
+        gc += q == 2 ? k : 0;
+      }
+      const bool small = size <= cs.small && slots <= cs.small_slots && ga <= kWgCapA &&
+                         gb <= kWgCapB && gc <= kWgCapC;
+      // the first level is the host's choice; later ones stop where the host would switch
+      s_go = size > 0 && (gen == 1 || (slots <= cs.max_slots && !small));
+      s_size = size;
+      s_slots = slots;
+    }
+    __syncthreads();
+    if (!s_go) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.state[0] = level;
+        a.state[1] = s_size;
+      }
+      break;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses)
+      pick(a.fcnt, (level + 2) % 3)[threadIdx.x * kCtlStride] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0)  // top-down slots explored (the bottom-up rule)
+      atomicAdd((unsigned long long *)(a.state + 4), (unsigned long long)s_slots);
+    ListSet out = a.fr[0];
+    out.items = (level & 1) ? a.fr[0].items : a.fr[1].items;
+    out.cnt = pick(a.fcnt, (level + 1) % 3);
+    stage_init(sf);
+    stage_init(sa);
+    for (int c = 0; c < d.nclass; ++c) {
+      const int cnt = list_count(in, c);
+      if (cnt > 0)
+        bfs_dispatch<true>(g, a, in, out, BfsStages{&sf, &sa}, c, pick(d.lanes, c), cnt, gtid,
+                           gthreads, level, nv, ns);
+    }
+    stage_flush(sf, out, a.nclass);
+    stage_flush(sa, a.active, a.nclass_act);
+    if (!coop_barrier(bar, gen * gridDim.x, abort)) break;
+    ++level;
+  }
+  block_flush(nv, ns, 0, 0, a.stats, ST_BFS_VERTS);
+}
+
 // ---- bottom-up levels (large frontiers) ----
 //
 // When the frontier holds a large part of the graph, a top-down level scans most slots of the
@@ -1849,11 +1958,14 @@ struct ServeArgs {
   int *claimed;                // device memory: entries claimed so far
   int *pub_seen;               // device memory: the largest published count read so far
   long long *poll_at;          // device memory: wall_clock64 of the last host read
-  long long *last_claim;       // device memory: wall_clock64 of the last claim by any workgroup
+  long long *last_claim;       // device memory: wall_clock64 of the last claim or finish
+  int *busy;                   // device memory: workgroups solving an instance now
   int qcap;
-  long long idle_ticks;        // ticks without any claim after which idle workgroups leave
+  long long idle_ticks;        // ticks without any claim or finish after which idle workgroups
+                               // leave (none may be busy)
 };
 constexpr long long kHostPollTicks = 500;  // 5 us at the 100 MHz wall clock
+constexpr int kServeCoreStride = 4;        // one serving workgroup in four stays while any works
 
 // Published entries, as one idle workgroup's thread sees them: the host's count when no
 // workgroup has read it for kHostPollTicks, else the last count read.
@@ -1897,16 +2009,27 @@ __global__ __launch_bounds__(kBlock) void k_serve_wg(L g, ServeArgs a) {
         c = old;
       }
       const long long now = (long long)wall_clock64();
-      if (q >= 0)
+      if (q >= 0) {
+        atomicAdd(a.busy, 1);
         __hip_atomic_fetch_max(a.last_claim, now, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      // the grid leaves together: an idle workgroup leaves once no workgroup has claimed an
-      // entry for idle_ticks, so requests that arrive one by one keep the whole grid, not only
-      // the workgroups that happened to be busy
-      const long long since = max(idle_from, __hip_atomic_load(a.last_claim, __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_AGENT));
-      if (q < 0 && (now - since > a.idle_ticks ||
-                    __hip_atomic_load(a.closed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)))
-        q = -2;
+      }
+      // Leaving.  A workgroup leaves after idle_ticks without work of its own, except the core
+      // (every kServeCoreStride-th workgroup), which leaves only when no workgroup is solving an
+      // instance and none has claimed or finished one for idle_ticks.  Requests arriving one by
+      // one, or beside a long instance, then always find the core, not only the workgroups that
+      // happened to be busy; and the idle rest does not hold registers that solves running
+      // alone beside the serving kernel can use.
+      if (q < 0) {
+        bool quiet = now - idle_from > a.idle_ticks;
+        if (quiet && blockIdx.x % kServeCoreStride == 0) {
+          const long long last = __hip_atomic_load(a.last_claim, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+          quiet = __hip_atomic_load(a.busy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+                  now - last > a.idle_ticks;
+        }
+        if (quiet || __hip_atomic_load(a.closed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+          q = -2;
+      }
       s_q = q;
     }
     __syncthreads();
@@ -1948,6 +2071,9 @@ __global__ __launch_bounds__(kBlock) void k_serve_wg(L g, ServeArgs a) {
                          __HIP_MEMORY_SCOPE_SYSTEM);
       __hip_atomic_store(&r->status, status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       __hip_atomic_store(&r->seq, q + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_fetch_max(a.last_claim, (long long)wall_clock64(), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      atomicSub(a.busy, 1);
     }
     block_flush(st.active, st.slots, st.pushes, st.relabels, a.b.stats, ST_ACTIVE);
     st = Stats{0, 0, 0, 0};
@@ -2013,6 +2139,10 @@ constexpr int kBatchGroupsPerCU = 4;         // at most this many batched workgr
 // counters: R-MAT 22 solves take 7.63 / 7.48 / 7.45 / 7.40 ms at 8 / 4 / 2 / 1 blocks, Genrmf 64
 // is level within its spread.
 constexpr int kBfsBlocksPerCU = 2;
+// Cooperative top-down levels (k_bfs_coop): workgroups of the launch, and the largest frontier
+// (estimated slots) they take; larger frontiers go to one grid-wide launch per level.
+constexpr int kCoopBlocks = 64;
+constexpr long long kCoopSlots = 1 << 17;
 
 template <class T>
 static mf_status dmalloc(T **p, size_t count) {
@@ -2116,6 +2246,7 @@ struct mf_context {
   hipEvent_t ev_solve[2] = {nullptr, nullptr};  // brackets of the solve loop's phases
   void *staging = nullptr;  // pinned host buffer for packed batches (grown on demand)
   size_t staging_bytes = 0;
+  bool cooperative = false;  // the device runs cooperative launches
 };
 
 namespace mf {
@@ -2559,6 +2690,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
 
   const int fmax = o.bfs_block_frontier > 0 ? o.bfs_block_frontier : kDefaultBfsFrontier;
   const int fmax_low = std::max(1, fmax / 2);
+  bool use_coop = ctx->cooperative && o.bfs_cooperative == 1;
   const bool adaptive = o.cycles_per_relabel == 0;
   const int K = adaptive ? kMaxAdaptiveCycles : o.cycles_per_relabel;
   const double ratio = o.relabel_time_ratio > 0 ? o.relabel_time_ratio : kDefaultRelabelRatio;
@@ -2651,6 +2783,8 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     }
     // ---- global relabel ----
     HIP_TRY(hipEventRecord(e0, st));
+    bool coop_failed = false;
+  relabel_again:
     k_bfs_init<<<vgrid, 256, 0, st>>>(ba, ccnt[0], ccnt[1]);
     k_bfs_seed<<<kgrid, 256, 0, st>>>(ba, G.src, G.snk, G.k);
     int level = 0, levels_total = 0;
@@ -2699,6 +2833,24 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
           ++bfs_bu_levels;
           explored += mf;
           next = level + 1;
+        } else if (use_coop && mf <= (double)kCoopSlots) {
+          // levels inside one cooperative launch until the frontier leaves the mid-size range
+          CoopStop cs;
+          cs.small = fmax_low;
+          cs.small_slots = ba.wg_slots / 2;
+          cs.max_slots = std::min<long long>(
+              kCoopSlots, (bu_allowed && !bu_always) ? (long long)(mu / kBottomUpAlpha / 2)
+                                                     : std::numeric_limits<long long>::max());
+          unsigned *bar = reinterpret_cast<unsigned *>(G.ctl + CTL_STATE + 6);
+          int *abort_flag = G.ctl + CTL_STATE + 7;
+          k_fill_int<<<1, 64, 0, st>>>(G.ctl + CTL_STATE + 6, 0, 2);
+          int lv = level;
+          void *args[] = {(void *)&lay, (void *)&ba, (void *)&bdisp, (void *)&lv,
+                          (void *)&cs,  (void *)&bar, (void *)&abort_flag};
+          HIP_TRY(hipLaunchCooperativeKernel((const void *)k_bfs_coop<L>, dim3(kCoopBlocks),
+                                             dim3(kBlock), args, 0, st));
+          next = -2;
+          bu_prev = false;
         } else {
           // top-down levels go kGridLevelBatch at a time between two read-backs; the levels after
           // the first stop by themselves (LevelStop) when the frontier empties, nears half the
@@ -2719,6 +2871,13 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_READ, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
+        if (next == -2) {
+          if (hstate[CTL_STATE + 7] != 0) {  // a grid barrier gave up: redo without them
+            coop_failed = true;
+            break;
+          }
+          next = hstate[CTL_STATE];
+        }
         if (next < 0) next = std::max(hstate[CTL_STATE + 2], hstate[CTL_STATE + 3]);
         bfs_grid_levels += next - level;
         level = next;
@@ -2726,7 +2885,11 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
         for (int c = 0; c < bcfg.nclass; ++c) size += hstate[ctl_cnt(CTL_FCNT, level % 3, c)];
         if (size <= fmax_low && frontier_slots(level) <= ba.wg_slots / 2 && fits_wg(level)) break;
       }
-      if (size == 0) break;
+      if (coop_failed || size == 0) break;
+    }
+    if (coop_failed && use_coop) {
+      use_coop = false;
+      goto relabel_again;
     }
     levels_total = level;
     bfs_levels += level;
@@ -2912,7 +3075,7 @@ static mf_status check_options(const mf_options &o) {
   if (o.cycles_per_relabel < 0 || o.max_rounds < 0 || o.bfs_block_frontier < 0)
     return MF_ERR_INVALID_ARGUMENT;
   if (o.bfs_direction < 0 || o.bfs_direction > 2 || o.profile_kernels < 0 ||
-      o.forward_hops < -1)
+      o.forward_hops < -1 || o.bfs_cooperative < 0 || o.bfs_cooperative > 1)
     return MF_ERR_INVALID_ARGUMENT;
   return MF_OK;
 }
@@ -3150,7 +3313,7 @@ struct ServeArena {
   int2 *dc = nullptr;
   unsigned char *vcls = nullptr;
   int *words = nullptr;  // [0] claimed, [1] published count seen, [2] overflow, [4..5] poll time,
-                         // [6..7] last claim time
+                         // [6..7] last claim or finish time, [8] busy workgroups
   unsigned long long *stats = nullptr;
   // host memory the GPU reads and writes uncached
   int *entries = nullptr;
@@ -3380,6 +3543,7 @@ mf_status serve_launch(ServeArena *a) {
   sa.pub_seen = a->words + 1;
   sa.poll_at = (long long *)(a->words + 4);
   sa.last_claim = (long long *)(a->words + 6);
+  sa.busy = a->words + 8;
   sa.qcap = a->qcap;
   sa.idle_ticks = kServeIdleTicks;
   const int nwg = std::max(1, kServeGroupsPerCU * a->ctx->num_cu);
@@ -3538,6 +3702,10 @@ mf_status mf_context_create(int device, mf_context **out) {
   mf_context *ctx = new mf_context();
   ctx->device = device;
   ctx->num_cu = prop.multiProcessorCount;
+  int coop = 0;
+  ctx->cooperative = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) ==
+                         hipSuccess &&
+                     coop != 0;
   snprintf(ctx->name, sizeof(ctx->name), "%s", prop.name);
   snprintf(ctx->arch, sizeof(ctx->arch), "%s", prop.gcnArchName);
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
