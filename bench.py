@@ -82,7 +82,9 @@ WORKLOADS = {
 # launches the timed run makes; cooperative BFS levels cost R-MAT 0.16 ms (7.50 vs 7.66 ms).
 # Genrmf 64: 156 trials, best 98.2 ms; then cooperative BFS levels for mid-size frontiers with the
 # one-workgroup kernel's limit lowered to 256 vertices: 93.9 against 99.2 ms (4 solves each,
-# interleaved).  Workloads not listed run the library defaults.
+# interleaved).  Started from these options, the autotuner kept them on both graphs (R-MAT 22: 138
+# trials, best 7.42 ms; Genrmf 64: 147 trials, best 93.7 ms).  Workloads not listed run the
+# library defaults.
 TUNED = {
     ("rmat22", "vc"): {"cycles_per_relabel": 12, "tile_lanes": 0, "bfs_block_frontier": 1024,
                        "relabel_time_ratio": 2.0, "bfs_direction": 0,
