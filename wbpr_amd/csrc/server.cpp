@@ -161,10 +161,11 @@ extern "C" int64_t mf_footprint_bytes(int32_t n, int64_t m) {
   //   slot of every arc 4m, offsets 4n) plus the raw CSR copy (8m + 4n) and the device build's
   //   arrays (five arc arrays 20m, in-offsets 4n, radix-sort buffers 8m): 72m + 12n
   // and what stays resident: the layout 36m + 4n, plus heights, excesses, stamps, four worklists
-  // and the degree class, 29n;
+  // and the degree class, 29n; the bottom-up levels' two lists of unreached vertices, 8n; and the
+  // map of changed residual lines of graphs of 2^24 slots or more, m/4;
   // plus control words, statistics and terminals (64 KiB).
   const int64_t peak = 72 * m + 12 * (int64_t)n;
-  const int64_t resident = 36 * m + 33 * (int64_t)n;
+  const int64_t resident = 36 * m + m / 4 + 41 * (int64_t)n;
   return std::max(peak, resident) + (64 << 10);
 }
 
