@@ -1161,13 +1161,9 @@ __device__ __forceinline__ void bu_vertex(const L &g, const BfsArgs &a, const Li
                                           int w, int c, int lane, int level, long long &nv,
                                           long long &ns) {
   constexpr int B = kBfsBatch;
-  // the vertex's height, excess and worklist class, read together
-  const int hw = a.h[w];
-  const int ew = a.e[w];
-  const int aw = a.acls ? a.acls[w] : 0;
   // reached already, or a source (tile-uniform: one address).  Sources are not scanned here:
   // the top-down levels check that no source reaches a sink
-  if (hw != a.H) return;
+  if (a.h[w] != a.H) return;
   if (lane == 0) ++nv;
   bool found = false;
 #pragma unroll
@@ -1196,7 +1192,10 @@ __device__ __forceinline__ void bu_vertex(const L &g, const BfsArgs &a, const Li
   }
   const bool mine = found && lane == 0;
   if (mine) a.h[w] = level + 1;
-  bfs_append(S, a, out, w, mine, mine && ew > 0, c, aw);
+  // excess and worklist class only for the reached: the unreached are most of the vertices a
+  // level examines, and the level is bound by its memory requests
+  const bool act = mine && a.e[w] > 0;
+  bfs_append(S, a, out, w, mine, act, c, (act && a.acls) ? a.acls[w] : 0);
   if (rem.items) staged_append(*SR, rem, w, !found && lane == 0, c, a.nclass);
 }
 
