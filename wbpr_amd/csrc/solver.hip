@@ -1243,13 +1243,14 @@ __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(L g, BfsArgs a, Dispat
   rem.cstride = kCtlStride;
   rem.ovf = a.flags + 1;
   long long nv = 0, ns = 0;
-  switch (pick(d.lanes, c)) {  // same lanes per vertex as the top-down levels
+  switch (pick(d.lanes, c)) {  // a quarter of the class's lanes (kBottomUpLanes) per vertex
 #define MF_BU_CASE(LANES, TT)                                                                  \
   case LANES:                                                                                  \
     bu_class<TT>(g, a, items, base, cnt, out, rem, S, &sr, c, tid / TT, nthreads / TT,         \
                  tid % TT, level, nv, ns);                                                     \
     break;
     MF_BU_CASE(1, 1)
+    MF_BU_CASE(4, 1)
     MF_BU_CASE(8, 2)
     MF_BU_CASE(16, 4)
     MF_BU_CASE(32, 8)
@@ -2132,6 +2133,10 @@ constexpr int kGraphChunkMax = 60;           // longest captured chunk
 constexpr int64_t kBottomUpAlpha = 14;
 constexpr int64_t kBottomUpBeta = 24;
 constexpr int64_t kBottomUpMinVertices = 1 << 16;
+// Lanes per class of the bottom-up levels (a tile of lanes / 4 BFS lanes, 1 for 4), for the
+// classes <= 8, <= 16, <= 32 and <= 1,024 slots.  R-MAT 22 relabels: 2.03 ms with the top-down
+// levels' 8 / 16 / 32 / 64, 1.81 ms with these; other splits within 1.79-1.81 ms.
+constexpr int kBottomUpLanes[4] = {4, 8, 16, 32};
 constexpr int kBatchGroupsPerCU = 4;         // at most this many batched workgroups per CU
 // Top-down grid levels: blocks per CU per class.  Most of a batch's levels after the first find
 // nothing to do (LevelStop), and every block of such a launch still starts and reads the
@@ -2592,8 +2597,12 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   const ClassConfig bcfg = class_config(MF_ALGO_VC, 0);
   const Dispatch bdisp = make_dispatch(bcfg, G.vc_pop, ctx->num_cu, G.giant_parts, kBfsBlocksPerCU);
   const int bgrid = bdisp.blk_begin[bcfg.nclass];
-  // bottom-up levels walk the nonzero-degree vertices in class order
-  const Dispatch budisp = make_dispatch(bcfg, G.bu_pop, ctx->num_cu, 0);
+  // bottom-up levels walk the nonzero-degree vertices in class order, with tiles half as wide as
+  // the top-down levels': a vertex stops at its first residual arc into the frontier, so most of
+  // a level's memory requests are per vertex, not per slot
+  ClassConfig bucfg = bcfg;
+  for (int c = 0; c < 4 && c < bucfg.nclass; ++c) bucfg.lanes[c] = kBottomUpLanes[c];
+  const Dispatch budisp = make_dispatch(bucfg, G.bu_pop, ctx->num_cu, 0);
   const int bugrid = budisp.blk_begin[bcfg.nclass];
   const bool bu_allowed = o.bfs_direction == 0 && G.nonzero >= kBottomUpMinVertices;
   const bool bu_always = o.bfs_direction == 2;
