@@ -913,7 +913,7 @@ __global__ void k_bfs_init(BfsArgs a, int *ccnt0, int *ccnt1) {
       a.flags[0] = 0;
       a.state[2] = 0;  // grid levels' marks (k_bfs_level)
       a.state[3] = 0;
-      a.state[4] = 0;  // top-down slots explored (a 64-bit counter in words 4 and 5)
+      a.state[4] = 0;  // slots explored by grid-wide levels (a 64-bit counter in words 4 and 5)
       a.state[5] = 0;
     }
   }
@@ -1211,11 +1211,31 @@ __device__ __forceinline__ void bu_class(const L &g, const BfsArgs &a, const int
 
 // walk_all: the level walks every nonzero-degree vertex (the first bottom-up level after a
 // top-down one, or no lists of unreached vertices are kept); otherwise those the level before
-// left unreached.
+// left unreached.  Bottom-up levels are issued a few at a time between two read-backs: a level
+// after the batch's first (first) runs only if the level before it ran (its mark, as in
+// LevelStop) and the frontier still holds at least stay vertices, the size below which the
+// host turns back to top-down levels; every block decides alike from the complete counters.
 template <class L>
 __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(L g, BfsArgs a, Dispatch d,
                                                           VertexOrder vo, int level,
-                                                          int walk_all) {
+                                                          int walk_all, int first, int stay) {
+  {
+    ListSet cur = a.fr[0];
+    cur.cnt = pick(a.fcnt, level % 3);
+    int size = 0;
+    long long slots = 0;
+    for (int c = 0; c < d.nclass; ++c) {
+      const int k = list_count(cur, c);
+      size += k;
+      slots += (long long)k * pick(a.cls_mean, c);
+    }
+    if (level != first && (a.state[2 + (level & 1)] != level || size == 0 || size < stay)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.state[2 + ((level + 1) & 1)] = level + 1;  // the next level may run
+      // estimated slots of the frontiers expanded so far (the bottom-up rule's input)
+      atomicAdd((unsigned long long *)(a.state + 4), (unsigned long long)slots);
+    }
+  }
   if (blockIdx.x == 0 && threadIdx.x < (unsigned)kNumClasses) {
     pick(a.fcnt, (level + 2) % 3)[threadIdx.x * kCtlStride] = 0;
     if (a.rem_items[0]) pick(a.rcnt, (level + 2) % 3)[threadIdx.x * kCtlStride] = 0;
@@ -2119,6 +2139,7 @@ constexpr int kMaxAdaptiveCycles = 1 << 14;  // cap of cycles per round in adapt
 constexpr int kDefaultBfsFrontier = 4096;    // one-workgroup BFS up to this frontier size
 constexpr int kWgBfsSlots = 32768;           // ... and up to about this many frontier slots
 constexpr int kGridLevelBatch = 4;           // top-down grid levels issued per read-back
+constexpr int kBottomUpBatch = 3;            // bottom-up levels issued per read-back
 constexpr int kDefaultForwardHops = 2;       // push forwarding: vertices a tile chains per cycle
 constexpr int kCycleChunk = 64;              // most cycles launched between two worklist checks
 // graphs of at least this many residual slots track the lines pushes change, so a re-solve
@@ -2796,7 +2817,6 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     k_bfs_init<<<vgrid, 256, 0, st>>>(ba, ccnt[0], ccnt[1]);
     k_bfs_seed<<<kgrid, 256, 0, st>>>(ba, G.src, G.snk, G.k);
     int level = 0, levels_total = 0;
-    double explored = 0;  // slots of the frontiers expanded by grid-wide levels
     auto frontier_slots = [&](int lv) {
       double sl = 0;
       for (int c = 0; c < bcfg.nclass; ++c)
@@ -2822,25 +2842,28 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       for (;;) {  // frontier too large for one workgroup: grid launches per level
         // frontier slots estimated from its class counts and the classes' mean degrees
         const double mf = frontier_slots(level);
-        long long td_explored;
-        std::memcpy(&td_explored, &hstate[CTL_STATE + 4], sizeof(td_explored));
-        const double mu = std::max(0.0, (double)G.S - explored - (double)td_explored);
+        long long explored;  // estimated slots of the frontiers the grid-wide levels expanded
+        std::memcpy(&explored, &hstate[CTL_STATE + 4], sizeof(explored));
+        const double mu = std::max(0.0, (double)G.S - (double)explored);
         if (bu_allowed) {
           if (!bottom_up && mf * kBottomUpAlpha > mu) bottom_up = true;
           else if (bottom_up && (int64_t)size * kBottomUpBeta < G.nonzero) bottom_up = false;
         }
         bottom_up = bottom_up || bu_always;
         int next;
+        bool was_bu = false;
         if (bottom_up) {
           // the first bottom-up level of a run walks every vertex; its unreached list, which the
           // next bottom-up level walks, starts empty
           if (!bu_prev && G.bu_rem)
             k_fill_int<<<1, 256, 0, st>>>(ba.rcnt[(level + 1) % 3], 0, kNumClasses * kCtlStride);
-          k_bfs_bottom_up<L><<<bugrid, kBlock, 0, st>>>(lay, ba, budisp, G.vo, level, !bu_prev);
+          const int stay = bu_always ? 1 : (int)((G.nonzero + kBottomUpBeta - 1) / kBottomUpBeta);
+          for (int q = 0; q < kBottomUpBatch; ++q)
+            k_bfs_bottom_up<L><<<bugrid, kBlock, 0, st>>>(lay, ba, budisp, G.vo, level + q,
+                                                          q == 0 && !bu_prev, level, stay);
           bu_prev = true;
-          ++bfs_bu_levels;
-          explored += mf;
-          next = level + 1;
+          was_bu = true;
+          next = -1;
         } else if (use_coop && mf <= (double)kCoopSlots) {
           // levels inside one cooperative launch until the frontier leaves the mid-size range
           CoopStop cs;
@@ -2887,6 +2910,7 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
           next = hstate[CTL_STATE];
         }
         if (next < 0) next = std::max(hstate[CTL_STATE + 2], hstate[CTL_STATE + 3]);
+        if (was_bu) bfs_bu_levels += next - level;
         bfs_grid_levels += next - level;
         level = next;
         size = 0;
