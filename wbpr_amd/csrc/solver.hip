@@ -2268,7 +2268,7 @@ struct mf_context {
   char name[256] = {0};
   char arch[64] = {0};
   std::vector<hipEvent_t> prof_ev;
-  hipEvent_t ev_solve[2] = {nullptr, nullptr};  // brackets of the solve loop's phases
+  hipEvent_t ev_solve[4] = {nullptr, nullptr, nullptr, nullptr};  // brackets of the solve's phases
   void *staging = nullptr;  // pinned host buffer for packed batches (grown on demand)
   size_t staging_bytes = 0;
   bool cooperative = false;  // the device runs cooperative launches
@@ -2769,7 +2769,8 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   const int vgrid = std::max(1, std::min((n + 255) / 256, 16 * ctx->num_cu));
   const int kgrid = std::max(1, std::min((G.k + 255) / 256, 16 * ctx->num_cu));
 
-  hipEvent_t e0 = ctx->ev_solve[0], e1 = ctx->ev_solve[1];
+  hipEvent_t e0 = ctx->ev_solve[0], e1 = ctx->ev_solve[1];    // global relabels
+  hipEvent_t ec0 = ctx->ev_solve[2], ec1 = ctx->ev_solve[3];  // cycle phases
   double kernel_ms = 0, relabel_ms = 0, prof_ms = 0;
   int64_t prof_launches = 0, cycles = 0, rounds = 0, bfs_levels = 0, bfs_grid_levels = 0,
           bfs_bu_levels = 0;
@@ -2805,6 +2806,41 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
   HIP_TRY(hipGetLastError());
 
   int hstate[CTL_SIZE];
+  // elapsed times of a cycle chunk's timed launches (half: which set of profiling events)
+  auto collect = [&](int half, int count) -> hipError_t {
+    for (int q = 0; q < count; ++q) {
+      float kms = 0;
+      const size_t pe = (size_t)half * 2 * kCycleChunk + (size_t)2 * q;
+      hipError_t err = hipEventElapsedTime(&kms, ctx->prof_ev[pe], ctx->prof_ev[pe + 1]);
+      if (err != hipSuccess) return err;
+      prof_ms += kms;
+    }
+    prof_launches += count;
+    return hipSuccess;
+  };
+  // A fixed cadence's round ends with its last chunk of cycles: the next global relabel is
+  // enqueued right behind it, without a read-back, and the round's timings are read at the
+  // relabel's first synchronisation (finish_cycles).
+  struct DeferredRound {
+    bool on = false;
+    int prev_half = 0, prev = 0, last_half = 0, last = 0, graph = 0;
+  } dfr;
+  auto finish_cycles = [&]() -> mf_status {
+    if (!dfr.on) return MF_OK;
+    dfr.on = false;
+    if (dfr.prev) HIP_TRY(collect(dfr.prev_half, dfr.prev));
+    if (dfr.last) HIP_TRY(collect(dfr.last_half, dfr.last));
+    if (dfr.graph && o.profile_kernels > 0) {
+      float gms = 0;
+      HIP_TRY(hipEventElapsedTime(&gms, e_graph[0], e_graph[1]));
+      prof_ms += gms;
+      prof_launches += dfr.graph;
+    }
+    float cms = 0;
+    HIP_TRY(hipEventElapsedTime(&cms, ec0, ec1));
+    kernel_ms += cms;
+    return MF_OK;
+  };
   for (;;) {
     if (rounds >= max_rounds) {
       status = MF_ERR_ITERATION_LIMIT;
@@ -2835,6 +2871,10 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_READ, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
+      {
+        const mf_status fs = finish_cycles();  // the cycles before this relabel have finished
+        if (fs != MF_OK) return fs;
+      }
       level = hstate[CTL_STATE];
       int size = hstate[CTL_STATE + 1];
       if (size == 0) break;
@@ -2944,21 +2984,10 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
     if (active == 0) break;
     // ---- push-relabel cycles, issued in chunks; the round ends early once the worklist is
     //      empty (checked between chunks with one small read-back) ----
-    HIP_TRY(hipEventRecord(e0, st));
+    HIP_TRY(hipEventRecord(ec0, st));
     const double tc0 = now_ms();
     int done = 0, nchunk = 0, pending = 0;  // pending: timed launches of the previous chunk
-    // elapsed times of a chunk's timed launches are read while the next chunk runs
-    auto collect = [&](int half, int count) -> hipError_t {
-      for (int q = 0; q < count; ++q) {
-        float kms = 0;
-        const size_t pe = (size_t)half * 2 * kCycleChunk + (size_t)2 * q;
-        hipError_t err = hipEventElapsedTime(&kms, ctx->prof_ev[pe], ctx->prof_ev[pe + 1]);
-        if (err != hipSuccess) return err;
-        prof_ms += kms;
-      }
-      prof_launches += count;
-      return hipSuccess;
-    };
+    // (elapsed times of a chunk's timed launches are read while the next chunk runs)
     while (done < K) {
       // chunk length: what the rest of the time budget allows at the last measured cycle time
       // (one read-back per chunk; short chunks when cycles are long, up to kCycleChunk)
@@ -3026,6 +3055,18 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       }
       HIP_TRY(hipGetLastError());
       done += chunk;
+      if (!adaptive && done >= K) {  // the round's last chunk: the next relabel follows at once
+        dfr.on = true;
+        dfr.prev_half = half ^ 1;
+        dfr.prev = pending;
+        dfr.last_half = half;
+        dfr.last = timed;
+        dfr.graph = graph_chunk;
+        pending = 0;
+        ++nchunk;
+        graph_launches += graph_chunk;
+        break;
+      }
       HIP_TRY(hipMemcpyAsync(hstate, G.ctl, sizeof(int) * CTL_READ, hipMemcpyDeviceToHost, st));
       if (pending) HIP_TRY(collect(half ^ 1, pending));
       pending = timed;
@@ -3043,17 +3084,24 @@ static mf_status run_solve(mf_context *ctx, DevGraph &G, const L &lay, mf_algo a
       for (int c = 0; c < cfg.nclass; ++c) left += hstate[ctl_cnt(CTL_CCNT, done % 3, c)];
       if (left == 0) break;
     }
-    if (pending) HIP_TRY(collect((nchunk - 1) & 1, pending));
-    HIP_TRY(hipEventRecord(e1, st));
-    HIP_TRY(hipEventSynchronize(e1));
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    kernel_ms += ms;
+    HIP_TRY(hipEventRecord(ec1, st));
+    if (!dfr.on) {
+      if (pending) HIP_TRY(collect((nchunk - 1) & 1, pending));
+      HIP_TRY(hipEventSynchronize(ec1));
+      float cms = 0;
+      HIP_TRY(hipEventElapsedTime(&cms, ec0, ec1));
+      kernel_ms += cms;
+    }
     cycles += done;
   }
   const int ggrid = std::max(1, std::min((G.k + 255) / 256, 4 * ctx->num_cu));
   k_gather_values<<<ggrid, 256, 0, st>>>(G.e, G.snk, G.k, G.values);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
+  {
+    const mf_status fs = finish_cycles();  // a round the loop left without a further relabel
+    if (fs != MF_OK) return fs;
+  }
   const double t1 = now_ms();
   unsigned long long hs[kStatSlot];
   long long v0 = 0;
@@ -3742,7 +3790,9 @@ mf_status mf_context_create(int device, mf_context **out) {
   snprintf(ctx->arch, sizeof(ctx->arch), "%s", prop.gcnArchName);
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->ev_solve[0]) != hipSuccess ||
-      hipEventCreate(&ctx->ev_solve[1]) != hipSuccess) {
+      hipEventCreate(&ctx->ev_solve[1]) != hipSuccess ||
+      hipEventCreate(&ctx->ev_solve[2]) != hipSuccess ||
+      hipEventCreate(&ctx->ev_solve[3]) != hipSuccess) {
     mf_context_destroy(ctx);
     return MF_ERR_DEVICE;
   }
