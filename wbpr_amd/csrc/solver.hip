@@ -123,20 +123,6 @@ __device__ __forceinline__ void scan_slots(const L &g, const int *h, int u, int2
   }
 }
 
-// Slots [lo, hi) of u, counted over its segments in order.
-template <int B, class L>
-__device__ __forceinline__ void scan_range(const L &g, const int *h, int u, int lo, int hi,
-                                           int first, int step, Cand &c) {
-  int acc = 0;
-#pragma unroll
-  for (int seg = 0; seg < L::kSeg; ++seg) {
-    const int2 r = g.range(u, seg);
-    const int len = r.y - r.x;
-    scan_seg<B>(g, h, u, seg, r.x + max(lo - acc, 0), r.x + min(hi - acc, len), first, step, c);
-    acc += len;
-  }
-}
-
 template <class L>
 __device__ __forceinline__ int slot_degree(const L &g, int u) {
   int deg = 0;
