@@ -83,8 +83,9 @@ WORKLOADS = {
 # Genrmf 64: 156 trials, best 98.2 ms; then cooperative BFS levels for mid-size frontiers with the
 # one-workgroup kernel's limit lowered to 256 vertices: 93.9 against 99.2 ms (4 solves each,
 # interleaved).  Started from these options, the autotuner kept them on both graphs (R-MAT 22: 138
-# trials, best 7.42 ms; Genrmf 64: 147 trials, best 93.7 ms).  Workloads not listed run the
-# library defaults.
+# trials, best 7.42 ms; Genrmf 64: 147 trials, best 93.7 ms).  Genrmf 64's tile widths then
+# narrowed to 4 / 4 / 8 / 16 lanes up to 1,024 slots: 92.0 against 94.7 ms (8 interleaved solves
+# each).  Workloads not listed run the library defaults.
 TUNED = {
     ("rmat22", "vc"): {"cycles_per_relabel": 12, "tile_lanes": 0, "bfs_block_frontier": 1024,
                        "relabel_time_ratio": 2.0, "bfs_direction": 0,
@@ -92,7 +93,7 @@ TUNED = {
                        "bfs_cooperative": 0},
     ("genrmf64", "vc"): {"cycles_per_relabel": 674, "tile_lanes": 0, "bfs_block_frontier": 256,
                          "relabel_time_ratio": 4.0, "bfs_direction": 0,
-                         "class_lanes": [4, 8, 16, 64, 256], "forward_hops": 2, "cycle_graphs": 0,
+                         "class_lanes": [4, 4, 8, 16, 256], "forward_hops": 2, "cycle_graphs": 0,
                          "bfs_cooperative": 1},
 }
 

@@ -52,6 +52,7 @@ const uint16_t kLanePresets[][5] = {
     {2, 4, 8, 32, 256},    // narrow: four slots in flight per lane cover each class bound
     {4, 8, 16, 64, 256},   // middle
     {1, 2, 8, 32, 256},    // low-degree vertices thread-centric
+    {4, 4, 8, 16, 256},    // narrow up to 1,024 slots (long-path graphs of low degree)
 };
 const float kRatios[] = {0.5f, 1.0f, 2.0f, 4.0f, 8.0f};
 const int32_t kForwardHops[] = {-1, 2, 8, 32};
