@@ -1029,7 +1029,7 @@ __device__ __forceinline__ bool coop_barrier(unsigned *bar, unsigned target, int
   __syncthreads();
   if (threadIdx.x == 0) {
     int ok = 1;
-    __threadfence();  // the block's level is visible device-wide before it arrives
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the block's level is visible first
     atomicAdd(bar, 1u);
     unsigned spins = 0;
     while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -1041,7 +1041,7 @@ __device__ __forceinline__ bool coop_barrier(unsigned *bar, unsigned target, int
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    __threadfence();  // every other block's level is visible here
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every other block's level is visible
     s_ok = ok;
   }
   __syncthreads();
