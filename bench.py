@@ -95,6 +95,12 @@ TUNED = {
                          "relabel_time_ratio": 4.0, "bfs_direction": 0,
                          "class_lanes": [4, 4, 8, 16, 256], "forward_hops": 2, "cycle_graphs": 0,
                          "bfs_cooperative": 1},
+    # Genrmf 128: the library defaults plus cooperative BFS levels from 256 frontier vertices on
+    # (559 against 708 ms without them; the autotuner, started there, kept them: 135 trials)
+    ("genrmf128", "vc"): {"cycles_per_relabel": 0, "tile_lanes": 0, "bfs_block_frontier": 256,
+                          "relabel_time_ratio": 2.0, "bfs_direction": 0,
+                          "class_lanes": [8, 16, 32, 64, 256], "forward_hops": 2, "cycle_graphs": 0,
+                          "bfs_cooperative": 1},
 }
 
 BATCH_K = 4096
