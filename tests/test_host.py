@@ -85,6 +85,18 @@ def test_snap_errors(mf, tmp_path):
     np.array([4, 2, 2, 1, 9, 0, 1, 2, 2, 2], dtype=np.int32).tofile(str(b))  # dst 9 >= n
     with pytest.raises(mf.MFError):
         mf.load_snap_bin(str(b))
+    # a header announcing more than the file holds fails before anything is allocated
+    np.array([1 << 30, 1 << 30, 1 << 30, 0, 1], dtype=np.int32).tofile(str(b))
+    with pytest.raises(mf.MFError) as ei:
+        mf.load_snap_bin(str(b))
+    assert ei.value.name == "MF_ERR_PARSE"
+    g = mf.load_snap_txt(os.path.join(GOLDEN, "snap_8.txt"))
+    g.with_terminals(0, 7).save_snap_bin(str(b))
+    whole = b.read_bytes()
+    b.write_bytes(whole[:-4])  # last offset missing
+    with pytest.raises(mf.MFError) as ei:
+        mf.load_snap_bin(str(b))
+    assert ei.value.name == "MF_ERR_PARSE"
 
 
 # ---- generators ---------------------------------------------------------------------------

@@ -220,6 +220,18 @@ mf_status mf_load_snap_bin(const char *path, mf_host_graph **out) {
     fclose(f);
     return MF_ERR_PARSE;
   }
+  // the arrays the header announces must be in the file: a truncated or foreign file fails here,
+  // before its header sizes the allocation
+  const long at = ftell(f);
+  if (at < 0 || fseek(f, 0, SEEK_END) != 0) {
+    fclose(f);
+    return MF_ERR_IO;
+  }
+  const long size = ftell(f);
+  if (size < at + (long)sizeof(int32_t) * ((long)mp + (long)n + 1) || fseek(f, at, SEEK_SET) != 0) {
+    fclose(f);
+    return MF_ERR_PARSE;
+  }
   mf_host_graph *g = host_graph_alloc(n, mp);
   if (!g) {
     fclose(f);
