@@ -386,6 +386,43 @@ def test_no_path_value_zero(mf, orc, ctx):
         assert r.value == 0
 
 
+DEGENERATE = {  # (n, tails, heads, capacities, s, t)
+    "no_arcs": (2, [], [], [], 0, 1),
+    "parallel_s_t": (2, [0, 0, 0], [1, 1, 1], [1, 2, 3], 0, 1),
+    "only_backwards_and_loops": (3, [1, 0, 1, 2], [0, 0, 1, 2], [4, 5, 6, 7], 0, 1),
+    "loop_on_path": (4, [0, 1, 1, 1, 3], [1, 3, 1, 2, 1], [5, 3, 7, 1, 2], 0, 3),
+    "zero_capacity": (3, [0, 0, 2], [1, 2, 1], [0, 0, 9], 0, 1),
+    "isolated_terminals": (6, [2, 3, 4], [3, 4, 2], [1, 1, 1], 0, 5),
+}
+
+
+def degenerate_graph(mf, name):
+    n, u, v, c, s, t = DEGENERATE[name]
+    return mf.Graph.from_arcs(n, np.array(u, dtype=np.int64), np.array(v, dtype=np.int64),
+                              np.array(c, dtype=np.int32), s, t)
+
+
+@pytest.mark.parametrize("name", sorted(DEGENERATE))
+def test_degenerate_graphs(mf, orc, ctx, name):
+    # no arcs at all, parallel arcs between the terminals, a source with only incoming arcs and
+    # self-loops, self-loops on the path, zero capacities, terminals cut off from every arc:
+    # every algorithm and layout, certified
+    g = degenerate_graph(mf, name)
+    value = orc.dinic_maxflow(g)
+    for algo in ALGOS:
+        for layout in LAYOUTS:
+            solve_and_certify(mf, orc, ctx, g, algo, layout, value)
+
+
+def test_degenerate_graphs_batched(mf, orc, ctx):
+    # the same graphs as one batch (one workgroup per instance), both cycle algorithms
+    graphs = [degenerate_graph(mf, k) for k in sorted(DEGENERATE)]
+    expect = [orc.dinic_maxflow(h) for h in graphs]
+    for algo in (0, 1):
+        vals, _ = ctx.batch_solve(graphs, batch_algo=algo)
+        assert vals.tolist() == expect, algo
+
+
 @pytest.mark.parametrize("algo", ["0", "1"])
 def test_cli_output_contract(golden, algo):
     # stdout lines parsed by the reference's bash_run.py:18-22
