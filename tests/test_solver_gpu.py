@@ -393,6 +393,8 @@ DEGENERATE = {  # (n, tails, heads, capacities, s, t)
     "loop_on_path": (4, [0, 1, 1, 1, 3], [1, 3, 1, 2, 1], [5, 3, 7, 1, 2], 0, 3),
     "zero_capacity": (3, [0, 0, 2], [1, 2, 1], [0, 0, 9], 0, 1),
     "isolated_terminals": (6, [2, 3, 4], [3, 4, 2], [1, 1, 1], 0, 5),
+    # the largest capacities an upload accepts: the source's out-capacity at the int32 limit
+    "int32_capacities": (4, [0, 1, 1, 2], [1, 3, 2, 3], [2**31 - 1, 2**31 - 6, 2**31 - 1, 3], 0, 3),
 }
 
 
@@ -405,8 +407,8 @@ def degenerate_graph(mf, name):
 @pytest.mark.parametrize("name", sorted(DEGENERATE))
 def test_degenerate_graphs(mf, orc, ctx, name):
     # no arcs at all, parallel arcs between the terminals, a source with only incoming arcs and
-    # self-loops, self-loops on the path, zero capacities, terminals cut off from every arc:
-    # every algorithm and layout, certified
+    # self-loops, self-loops on the path, zero capacities, terminals cut off from every arc,
+    # capacities at the int32 limit: every algorithm and layout, certified
     g = degenerate_graph(mf, name)
     value = orc.dinic_maxflow(g)
     for algo in ALGOS:
