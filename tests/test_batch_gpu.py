@@ -5,7 +5,7 @@ import time
 import numpy as np
 import pytest
 
-from conftest import generated_graph
+from conftest import DEGENERATE, degenerate_graph, generated_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -104,6 +104,24 @@ def test_server_continuous_batching(mf, ctx, golden):
         st = srv.stats()
         assert st["completed"] == len(inst) and st["submitted"] == len(inst)
         assert st["arena_bytes"] == 0 and 0 < st["arena_peak_bytes"] <= 64 << 20
+    finally:
+        srv.close()
+
+
+def test_server_degenerate_graphs(mf, orc, ctx):
+    # graphs without arcs, with only loops and backward arcs, zero capacities or int32-limit
+    # capacities, served from the arena by the persistent kernel: every value equal to Dinic's
+    graphs = [degenerate_graph(mf, k) for k in sorted(DEGENERATE)]
+    expect = [orc.dinic_maxflow(g) for g in graphs]
+    srv = mf.Server(ctx, hbm_budget_bytes=16 << 20, max_inflight_launches=2)
+    try:
+        ids = {srv.submit(g): v for g, v in zip(graphs, expect)}
+        t0 = time.time()
+        while ids and time.time() - t0 < 60:
+            srv.progress(wait=True)
+            for rid in srv.completed():
+                assert srv.result(rid)[0] == ids.pop(rid)
+        assert not ids
     finally:
         srv.close()
 

@@ -8,7 +8,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, ROOT, fixture_graph, generated_graph
+from conftest import DEGENERATE, GOLDEN, ROOT, degenerate_graph, fixture_graph, generated_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -384,24 +384,6 @@ def test_no_path_value_zero(mf, orc, ctx):
     for algo in ALGOS:
         r = ctx.solve(g, mf.ALGOS[algo])
         assert r.value == 0
-
-
-DEGENERATE = {  # (n, tails, heads, capacities, s, t)
-    "no_arcs": (2, [], [], [], 0, 1),
-    "parallel_s_t": (2, [0, 0, 0], [1, 1, 1], [1, 2, 3], 0, 1),
-    "only_backwards_and_loops": (3, [1, 0, 1, 2], [0, 0, 1, 2], [4, 5, 6, 7], 0, 1),
-    "loop_on_path": (4, [0, 1, 1, 1, 3], [1, 3, 1, 2, 1], [5, 3, 7, 1, 2], 0, 3),
-    "zero_capacity": (3, [0, 0, 2], [1, 2, 1], [0, 0, 9], 0, 1),
-    "isolated_terminals": (6, [2, 3, 4], [3, 4, 2], [1, 1, 1], 0, 5),
-    # the largest capacities an upload accepts: the source's out-capacity at the int32 limit
-    "int32_capacities": (4, [0, 1, 1, 2], [1, 3, 2, 3], [2**31 - 1, 2**31 - 6, 2**31 - 1, 3], 0, 3),
-}
-
-
-def degenerate_graph(mf, name):
-    n, u, v, c, s, t = DEGENERATE[name]
-    return mf.Graph.from_arcs(n, np.array(u, dtype=np.int64), np.array(v, dtype=np.int64),
-                              np.array(c, dtype=np.int32), s, t)
 
 
 @pytest.mark.parametrize("name", sorted(DEGENERATE))
