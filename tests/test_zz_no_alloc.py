@@ -4,7 +4,7 @@ static scratch array (no device allocation).  Two calls in a row, the second on 
 import numpy as np
 import pytest
 
-from conftest import generated_graph
+from conftest import DEGENERATE, degenerate_graph, generated_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -39,3 +39,14 @@ def test_noalloc_limits(mf, ctx, golden):
         ctx.solve_noalloc(big)
     assert ei.value.name == "MF_ERR_OVERFLOW"
     assert ctx.solve_noalloc(g).value == v  # still usable afterwards
+
+
+@pytest.mark.parametrize("name", sorted(DEGENERATE))
+def test_noalloc_degenerate_graphs(mf, orc, ctx, name):
+    # graphs without arcs, with only loops and backward arcs, zero or int32-limit capacities,
+    # after larger graphs left their contents in the scratch array
+    g = degenerate_graph(mf, name)
+    value = orc.dinic_maxflow(g)
+    print(f"noalloc {name}", flush=True)
+    for algo in (0, 1):
+        check_pair(mf, orc, ctx, g, value, algo)
