@@ -228,11 +228,13 @@ mf_status mf_graph_solve(mf_context *ctx, mf_graph *g, mf_algo algo, const mf_op
                          mf_result *res);
 /* Flow on every input arc (cap - residual), m entries, after a solve.  The flow is a maximum
  * preflow; with return_excess != 0 the excess stranded at vertices that cannot reach t is sent
- * back to s on the host so the result satisfies conservation everywhere except s and t. */
+ * back to s on the host so the result satisfies conservation everywhere except s and t.
+ * flow_out may be NULL for a graph without arcs. */
 mf_status mf_graph_read_flow(mf_context *ctx, mf_graph *g, int32_t *flow_out, int32_t return_excess);
 /* Host utility: turns a preflow on g (flow[m], excess >= 0 everywhere but s) into a flow of the
  * same value by cancelling flow cycles and returning every stranded excess to s along
- * flow-carrying arcs.  flow is modified in place.  Used by mf_graph_read_flow. */
+ * flow-carrying arcs.  flow is modified in place (NULL allowed when g has no arcs).  Used by
+ * mf_graph_read_flow. */
 mf_status mf_flow_return_excess(const mf_csr *g, int32_t *flow);
 /* Min-cut side after a solve: side[v] = 1 if v can reach t in the residual graph, else 0. */
 mf_status mf_graph_read_cut(mf_context *ctx, mf_graph *g, uint8_t *sink_side);

@@ -421,6 +421,21 @@ def test_cli_output_contract(golden, algo):
     assert "Certificate: ok" in r.stdout
 
 
+@pytest.mark.parametrize("name", ["no_arcs", "parallel_s_t", "int32_capacities"])
+def test_cli_degenerate_graphs(mf, orc, tmp_path, name):
+    # the CLI on DIMACS files of degenerate graphs: the value line and an exit status of 0
+    g = degenerate_graph(mf, name)
+    path = str(tmp_path / "g.max")
+    g.save_dimacs(path)
+    for algo in ("0", "1"):
+        r = subprocess.run([os.path.join(ROOT, "wbpr_amd", "bin", "maxflow"), "-v", "2", "-f", path,
+                            "-a", algo, "-c"], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        m = re.search(r"implementation is (\d+), (\d+)", r.stdout)
+        assert m and int(m.group(1)) == orc.dinic_maxflow(g), r.stdout
+        assert "Certificate: ok" in r.stdout
+
+
 @pytest.mark.parametrize("layout", ["0", "1", "2"])
 def test_cli_flow_and_cut_files(mf, orc, golden, tmp_path, layout):
     # -o: the DIMACS solution lines of OutputFlow (network/DIMACS/washington/washington.c:1333-1399):

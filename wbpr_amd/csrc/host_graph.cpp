@@ -877,7 +877,7 @@ void mf_layout_free(mf_layout_arrays *arr) {
 }  // extern "C"
 
 extern "C" mf_status mf_flow_return_excess(const mf_csr *g, int32_t *flow) {
-  if (!flow) return MF_ERR_INVALID_ARGUMENT;
+  if (!g || (!flow && g->m > 0)) return MF_ERR_INVALID_ARGUMENT;
   mf_status st = mf::validate_csr(g);
   if (st != MF_OK) return st;
   mf::return_excess_to_source(g, flow);

@@ -3850,7 +3850,7 @@ mf_status mf_graph_solve(mf_context *ctx, mf_graph *g, mf_algo algo, const mf_op
 
 mf_status mf_graph_read_flow(mf_context *ctx, mf_graph *mg, int32_t *flow_out,
                              int32_t return_excess) {
-  if (!ctx || !mg || !flow_out) return MF_ERR_INVALID_ARGUMENT;
+  if (!ctx || !mg || (!flow_out && mg->g.m > 0)) return MF_ERR_INVALID_ARGUMENT;
   DevGraph &G = mg->g;
   if (!G.solved) return MF_ERR_INVALID_ARGUMENT;
   HIP_TRY(hipSetDevice(ctx->device));
