@@ -294,7 +294,8 @@ mf_status mf_autotune(mf_context *ctx, mf_graph *g, mf_algo algo, const mf_optio
  *   - *excess_total receives e[t], the flow value.  The reference's Excess_total is the preflow's
  *     total minus the excess its global relabel found stranded (push_relabel.cu:114-119,
  *     global_relabel.cu:143-177), which equals e[t] only once every stranded unit is counted;
- *     here stranded excess stays where it is (a maximum preflow) and the value is reported. */
+ *     here stranded excess stays where it is (a maximum preflow) and the value is reported.
+ * With E == 0 the per-arc arrays may be NULL. */
 mf_status mf_rcsr_push_relabel(mf_context *ctx, int algo_type, int V, int E, int source, int sink,
                                int *cpu_height, int *cpu_excess_flow,
                                const int *cpu_offsets, const int *cpu_destinations,

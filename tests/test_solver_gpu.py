@@ -351,6 +351,23 @@ def test_rcsr_reference_entry_point(mf, orc, ctx, golden):
         assert np.all(ff >= 0) and np.all(bf >= 0) and np.array_equal(ff + bf, g.cap)
         flow = bf.copy()
         assert orc.certify(g, flow, value, preflow=True)[0] == "ok"
+    # no arcs at all: value 0, nothing to write back
+    g = degenerate_graph(mf, "no_arcs")
+    roff, rdst, fidx = orc.rcsr_build(g)
+    z = np.zeros(0, np.int32)
+    for algo in (0, 1):
+        h = np.zeros(g.n, np.int32)
+        e = np.zeros(g.n, np.int32)
+        et, r = ctx.rcsr_push_relabel(algo, g.n, 0, g.s, g.t, h, e, g.offsets, z, z, z.copy(),
+                                      z.copy(), roff, rdst, fidx)
+        assert et == 0 and r.value == 0 and not e.any()
+        # ... and with NULL for every per-arc array, as a C caller may pass them
+        et = mf.C.c_int(-1)
+        mf._check(mf._lib.mf_rcsr_push_relabel(ctx._p, algo, g.n, 0, g.s, g.t, mf._ptr(h),
+                                               mf._ptr(e), mf._ptr(g.offsets), None, None, None,
+                                               None, mf._ptr(roff), None, None, mf.C.byref(et),
+                                               None, None), "rcsr_push_relabel")
+        assert et.value == 0 and not e.any()
 
 
 def test_error_paths(mf, ctx, golden):

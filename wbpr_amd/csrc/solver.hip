@@ -4032,9 +4032,11 @@ mf_status mf_rcsr_push_relabel(mf_context *ctx, int algo_type, int V, int E, int
                                int *cpu_fflows, int *cpu_bflows, const int *cpu_roffsets,
                                const int *cpu_rdestinations, const int *cpu_flow_idx,
                                int *excess_total, const mf_options *opts, mf_result *res) {
-  if (!ctx || !cpu_height || !cpu_excess_flow || !cpu_offsets || !cpu_destinations ||
-      !cpu_capacities || !cpu_fflows || !cpu_bflows || !cpu_roffsets || !cpu_rdestinations ||
-      !cpu_flow_idx || !excess_total || (algo_type != 0 && algo_type != 1))
+  // the per-arc arrays may be NULL when there are no arcs
+  if (!ctx || !cpu_height || !cpu_excess_flow || !cpu_offsets || !cpu_roffsets || !excess_total ||
+      (E > 0 && (!cpu_destinations || !cpu_capacities || !cpu_fflows || !cpu_bflows ||
+                 !cpu_rdestinations || !cpu_flow_idx)) ||
+      (algo_type != 0 && algo_type != 1))
     return MF_ERR_INVALID_ARGUMENT;
   mf_csr c;
   c.n = V;
@@ -4097,11 +4099,17 @@ mf_status mf_rcsr_push_relabel(mf_context *ctx, int algo_type, int V, int E, int
                         (cpu_roffsets[v + 1] - cpu_roffsets[v]);
   if ((st = set_vertex_classes(G, degree, s)) != MF_OK) return fail(st);
   G.max_src_deg = degree[(size_t)source];
-  bool ok = hipMemcpyAsync(G.off, cpu_offsets, sizeof(int) * (V + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
-            hipMemcpyAsync(G.fdc, fdc.data(), sizeof(int2) * E, hipMemcpyHostToDevice, s) == hipSuccess &&
-            hipMemcpyAsync(G.roff, cpu_roffsets, sizeof(int) * (V + 1), hipMemcpyHostToDevice, s) == hipSuccess &&
-            hipMemcpyAsync(G.rdi, rdi.data(), sizeof(int2) * E, hipMemcpyHostToDevice, s) == hipSuccess &&
-            hipMemcpyAsync(G.cap0, cpu_capacities, sizeof(int) * E, hipMemcpyHostToDevice, s) == hipSuccess &&
+  // copies of the per-arc arrays only when there are arcs (their pointers may be NULL then)
+  auto put = [&](void *dst, const void *src, size_t bytes) {
+    return bytes == 0 || hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) == hipSuccess;
+  };
+  auto get = [&](void *dst, const void *src, size_t bytes) {
+    return bytes == 0 || hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+  };
+  bool ok = put(G.off, cpu_offsets, sizeof(int) * (V + 1)) &&
+            put(G.fdc, fdc.data(), sizeof(int2) * E) &&
+            put(G.roff, cpu_roffsets, sizeof(int) * (V + 1)) &&
+            put(G.rdi, rdi.data(), sizeof(int2) * E) && put(G.cap0, cpu_capacities, sizeof(int) * E) &&
             hipMemcpyAsync(G.src, &source, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(G.snk, &sink, sizeof(int), hipMemcpyHostToDevice, s) == hipSuccess &&
             hipStreamSynchronize(s) == hipSuccess;
@@ -4111,10 +4119,8 @@ mf_status mf_rcsr_push_relabel(mf_context *ctx, int algo_type, int V, int E, int
   if (res) *res = r;
   if (st == MF_OK) {
     std::vector<int2> fd((size_t)E);
-    ok = hipMemcpy(fd.data(), G.fdc, sizeof(int2) * E, hipMemcpyDeviceToHost) == hipSuccess &&
-         hipMemcpy(cpu_bflows, G.bcf, sizeof(int) * E, hipMemcpyDeviceToHost) == hipSuccess &&
-         hipMemcpy(cpu_height, G.h, sizeof(int) * V, hipMemcpyDeviceToHost) == hipSuccess &&
-         hipMemcpy(cpu_excess_flow, G.e, sizeof(int) * V, hipMemcpyDeviceToHost) == hipSuccess;
+    ok = get(fd.data(), G.fdc, sizeof(int2) * E) && get(cpu_bflows, G.bcf, sizeof(int) * E) &&
+         get(cpu_height, G.h, sizeof(int) * V) && get(cpu_excess_flow, G.e, sizeof(int) * V);
     if (!ok) return fail(MF_ERR_DEVICE);
     for (int i = 0; i < E; ++i) cpu_fflows[i] = fd[(size_t)i].y;
     *excess_total = cpu_excess_flow[sink];
