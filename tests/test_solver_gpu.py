@@ -415,6 +415,30 @@ def test_degenerate_graphs(mf, orc, ctx, name):
             solve_and_certify(mf, orc, ctx, g, algo, layout, value)
 
 
+def test_no_arcs_null_arrays(mf, ctx):
+    # a C caller's graph without arcs may carry NULL arc arrays: upload and the one-shot solve
+    # take it, on every layout and algorithm, and report 0
+    g = degenerate_graph(mf, "no_arcs")
+    c = g.csr()
+    c.dst = None
+    c.cap = None
+    for layout in (mf.LAYOUT_RCSR, mf.LAYOUT_BCSR, mf.LAYOUT_BCSR2):
+        p = mf._vp()
+        mf._check(mf._lib.mf_graph_upload(ctx._p, mf.C.byref(c), layout, mf.C.byref(p)), "upload")
+        try:
+            for algo in (mf.ALGO_TC, mf.ALGO_VC):
+                r = mf._Result()
+                mf._check(mf._lib.mf_graph_solve(ctx._p, p, algo, None, mf.C.byref(r)), "solve")
+                assert r.value == 0
+                mf._check(mf._lib.mf_graph_read_flow(ctx._p, p, None, 1), "read_flow")
+        finally:
+            mf._lib.mf_graph_destroy(ctx._p, p)
+        r = mf._Result()
+        mf._check(mf._lib.mf_solve(ctx._p, mf.C.byref(c), mf.ALGO_VC, layout, None, mf.C.byref(r),
+                                   None), "mf_solve")
+        assert r.value == 0
+
+
 def test_degenerate_graphs_batched(mf, orc, ctx):
     # the same graphs as one batch (one workgroup per instance), both cycle algorithms
     graphs = [degenerate_graph(mf, k) for k in sorted(DEGENERATE)]

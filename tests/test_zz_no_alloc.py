@@ -50,3 +50,15 @@ def test_noalloc_degenerate_graphs(mf, orc, ctx, name):
     print(f"noalloc {name}", flush=True)
     for algo in (0, 1):
         check_pair(mf, orc, ctx, g, value, algo)
+
+
+def test_noalloc_no_arcs_null_arrays(mf, ctx):
+    # a graph without arcs whose arc arrays are NULL, as a C caller may pass it
+    c = degenerate_graph(mf, "no_arcs").csr()
+    c.dst = None
+    c.cap = None
+    for algo in (mf.ALGO_TC, mf.ALGO_VC):
+        r = mf._Result()
+        mf._check(mf._lib.mf_solve_noalloc(ctx._p, mf.C.byref(c), algo, mf.LAYOUT_BCSR, None,
+                                           mf.C.byref(r), None), "mf_solve_noalloc")
+        assert r.value == 0

@@ -2497,9 +2497,11 @@ static mf_status upload(mf_context *ctx, const mf_csr *c, const int32_t *src, co
     free_raw();
     return rc;
   }
+  // (a graph without arcs may pass NULL arc arrays: nothing to copy then)
   if (hipMemcpyAsync(d_off, c->offsets, sizeof(int) * (c->n + 1), hipMemcpyHostToDevice, st) ||
-      hipMemcpyAsync(d_dst, c->dst, sizeof(int) * c->m, hipMemcpyHostToDevice, st) ||
-      hipMemcpyAsync(d_cap, cap, sizeof(int) * c->m, hipMemcpyHostToDevice, st)) {
+      (c->m > 0 &&
+       (hipMemcpyAsync(d_dst, c->dst, sizeof(int) * c->m, hipMemcpyHostToDevice, st) ||
+        hipMemcpyAsync(d_cap, cap, sizeof(int) * c->m, hipMemcpyHostToDevice, st)))) {
     free_raw();
     return MF_ERR_DEVICE;
   }
