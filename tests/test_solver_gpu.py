@@ -439,13 +439,24 @@ def test_no_arcs_null_arrays(mf, ctx):
         assert r.value == 0
 
 
-def test_degenerate_graphs_batched(mf, orc, ctx):
-    # the same graphs as one batch (one workgroup per instance), both cycle algorithms
+def test_degenerate_graphs_batched(mf, orc, ctx, golden):
+    # the same graphs as one batch, alone and interleaved with ordinary instances, in both batch
+    # modes (a workgroup per instance; the grid-wide kernels over the union) and both algorithms
     graphs = [degenerate_graph(mf, k) for k in sorted(DEGENERATE)]
     expect = [orc.dinic_maxflow(h) for h in graphs]
-    for algo in (0, 1):
-        vals, _ = ctx.batch_solve(graphs, batch_algo=algo)
-        assert vals.tolist() == expect, algo
+    mixed, mexpect = [], []
+    for i, name in enumerate(["rlg32_s7", "genrmf12_s2", "rmat10_s5", "bip_500_700_s5"]):
+        g, v = generated_graph(mf, golden, name)
+        mixed += [g, graphs[i]]
+        mexpect += [v, expect[i]]
+    mixed += graphs[4:]
+    mexpect += expect[4:]
+    for mode in (0, 1):
+        for algo in (0, 1):
+            vals, _ = ctx.batch_solve(graphs, batch_algo=algo, batch_mode=mode)
+            assert vals.tolist() == expect, (mode, algo)
+            vals, _ = ctx.batch_solve(mixed, batch_algo=algo, batch_mode=mode)
+            assert vals.tolist() == mexpect, (mode, algo)
 
 
 @pytest.mark.parametrize("algo", ["0", "1"])
