@@ -81,6 +81,39 @@ def test_random_multigraphs(mf, orc, ctx, seed):
             solve_and_certify(mf, orc, ctx, g, algo, layout, expect)
 
 
+def test_random_small_graphs_random_options(mf, orc, ctx):
+    # 600 small random multigraphs (2-64 vertices, sparse to dense, capacities from {0, 1} up to
+    # the int32 limit over few source arcs, random terminals), each on a random algorithm, layout
+    # and set of launch options: value equal to Dinic's, flow certified
+    rng = np.random.default_rng(4242)
+    for case in range(600):
+        n = int(rng.integers(2, 65))
+        m = int(rng.integers(0, 6 * n + 1))
+        u = rng.integers(0, n, size=m)
+        v = rng.integers(0, n, size=m)
+        s, t = (int(x) for x in rng.choice(n, size=2, replace=False))
+        top = int(rng.choice([1, 10, 1000, 2**31 - 1]))
+        cap = rng.integers(0, top, size=m, endpoint=True).astype(np.int64)
+        out_s = np.nonzero(u == s)[0]
+        if cap[out_s].sum() > 2**31 - 1:  # every excess must fit int32: scale the source arcs
+            cap[out_s] = cap[out_s] // (len(out_s) + 1)
+        g = mf.Graph.from_arcs(n, u, v, cap.astype(np.int32), s, t)
+        algo = str(rng.choice(ALGOS))
+        layout = str(rng.choice(LAYOUTS))
+        opts = dict(cycles_per_relabel=int(rng.choice([0, 1, 2, 7, 100])),
+                    bfs_block_frontier=int(rng.choice([0, 1, 8, 4096])),
+                    bfs_direction=int(rng.choice([0, 1, 2])),
+                    forward_hops=int(rng.choice([-1, 0, 3])),
+                    bfs_cooperative=int(rng.choice([0, 1])))
+        if algo == "vc":
+            opts["tile_lanes"] = int(rng.choice([0, 1, 8, 16, 32, 64, 256]))
+        expect = orc.dinic_maxflow(g)
+        try:
+            solve_and_certify(mf, orc, ctx, g, algo, layout, expect, **opts)
+        except AssertionError as err:
+            raise AssertionError(f"case {case}: n={n} m={m} s={s} t={t} {algo}/{layout} {opts}: {err}")
+
+
 def test_launch_geometry_stress(mf, orc, ctx, golden):
     # repeated solves with varied cadence, tile width and BFS hand-over: the lock-free
     # schedule differs every time, the value may not
