@@ -81,23 +81,27 @@ def test_random_multigraphs(mf, orc, ctx, seed):
             solve_and_certify(mf, orc, ctx, g, algo, layout, expect)
 
 
+def random_small_graph(mf, rng):
+    n = int(rng.integers(2, 65))
+    m = int(rng.integers(0, 6 * n + 1))
+    u = rng.integers(0, n, size=m)
+    v = rng.integers(0, n, size=m)
+    s, t = (int(x) for x in rng.choice(n, size=2, replace=False))
+    top = int(rng.choice([1, 10, 1000, 2**31 - 1]))
+    cap = rng.integers(0, top, size=m, endpoint=True).astype(np.int64)
+    out_s = np.nonzero(u == s)[0]
+    if cap[out_s].sum() > 2**31 - 1:  # every excess must fit int32: scale the source arcs
+        cap[out_s] = cap[out_s] // (len(out_s) + 1)
+    return mf.Graph.from_arcs(n, u, v, cap.astype(np.int32), s, t)
+
+
 def test_random_small_graphs_random_options(mf, orc, ctx):
     # 600 small random multigraphs (2-64 vertices, sparse to dense, capacities from {0, 1} up to
     # the int32 limit over few source arcs, random terminals), each on a random algorithm, layout
     # and set of launch options: value equal to Dinic's, flow certified
     rng = np.random.default_rng(4242)
     for case in range(600):
-        n = int(rng.integers(2, 65))
-        m = int(rng.integers(0, 6 * n + 1))
-        u = rng.integers(0, n, size=m)
-        v = rng.integers(0, n, size=m)
-        s, t = (int(x) for x in rng.choice(n, size=2, replace=False))
-        top = int(rng.choice([1, 10, 1000, 2**31 - 1]))
-        cap = rng.integers(0, top, size=m, endpoint=True).astype(np.int64)
-        out_s = np.nonzero(u == s)[0]
-        if cap[out_s].sum() > 2**31 - 1:  # every excess must fit int32: scale the source arcs
-            cap[out_s] = cap[out_s] // (len(out_s) + 1)
-        g = mf.Graph.from_arcs(n, u, v, cap.astype(np.int32), s, t)
+        g = random_small_graph(mf, rng)
         algo = str(rng.choice(ALGOS))
         layout = str(rng.choice(LAYOUTS))
         opts = dict(cycles_per_relabel=int(rng.choice([0, 1, 2, 7, 100])),
@@ -111,7 +115,22 @@ def test_random_small_graphs_random_options(mf, orc, ctx):
         try:
             solve_and_certify(mf, orc, ctx, g, algo, layout, expect, **opts)
         except AssertionError as err:
-            raise AssertionError(f"case {case}: n={n} m={m} s={s} t={t} {algo}/{layout} {opts}: {err}")
+            raise AssertionError(f"case {case}: n={g.n} m={g.m} s={g.s} t={g.t} {algo}/{layout} "
+                                 f"{opts}: {err}")
+
+
+def test_random_small_graphs_batched(mf, orc, ctx):
+    # 12 batches of 2-80 small random multigraphs, each batch with random batch options (both
+    # algorithms, both modes, fixed and adaptive cadence): every value equal to Dinic's
+    rng = np.random.default_rng(777)
+    for case in range(12):
+        graphs = [random_small_graph(mf, rng) for _ in range(int(rng.integers(2, 81)))]
+        expect = [orc.dinic_maxflow(g) for g in graphs]
+        opts = dict(batch_algo=int(rng.integers(0, 2)), batch_mode=int(rng.integers(0, 2)),
+                    cycles_per_relabel=int(rng.choice([0, 1, 5, 50])),
+                    forward_hops=int(rng.choice([-1, 0, 2])))
+        vals, _ = ctx.batch_solve(graphs, **opts)
+        assert vals.tolist() == expect, (case, opts)
 
 
 def test_launch_geometry_stress(mf, orc, ctx, golden):
