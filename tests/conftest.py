@@ -71,3 +71,17 @@ def degenerate_graph(mf, name):
     n, u, v, c, s, t = DEGENERATE[name]
     return mf.Graph.from_arcs(n, np.array(u, dtype=np.int64), np.array(v, dtype=np.int64),
                               np.array(c, dtype=np.int32), s, t)
+
+
+def random_small_graph(mf, rng):
+    n = int(rng.integers(2, 65))
+    m = int(rng.integers(0, 6 * n + 1))
+    u = rng.integers(0, n, size=m)
+    v = rng.integers(0, n, size=m)
+    s, t = (int(x) for x in rng.choice(n, size=2, replace=False))
+    top = int(rng.choice([1, 10, 1000, 2**31 - 1]))
+    cap = rng.integers(0, top, size=m, endpoint=True).astype(np.int64)
+    out_s = np.nonzero(u == s)[0]
+    if cap[out_s].sum() > 2**31 - 1:  # every excess must fit int32: scale the source arcs
+        cap[out_s] = cap[out_s] // (len(out_s) + 1)
+    return mf.Graph.from_arcs(n, u, v, cap.astype(np.int32), s, t)

@@ -5,7 +5,7 @@ import time
 import numpy as np
 import pytest
 
-from conftest import DEGENERATE, degenerate_graph, generated_graph
+from conftest import DEGENERATE, degenerate_graph, generated_graph, random_small_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -116,6 +116,29 @@ def test_server_degenerate_graphs(mf, orc, ctx):
     srv = mf.Server(ctx, hbm_budget_bytes=16 << 20, max_inflight_launches=2)
     try:
         ids = {srv.submit(g): v for g, v in zip(graphs, expect)}
+        t0 = time.time()
+        while ids and time.time() - t0 < 60:
+            srv.progress(wait=True)
+            for rid in srv.completed():
+                assert srv.result(rid)[0] == ids.pop(rid)
+        assert not ids
+    finally:
+        srv.close()
+
+
+def test_server_random_small_graphs(mf, orc, ctx):
+    # 80 small random multigraphs (capacities up to the int32 limit, degenerate ones among them)
+    # submitted in bursts while earlier ones run: every served value equal to Dinic's
+    rng = np.random.default_rng(31337)
+    graphs = [random_small_graph(mf, rng) for _ in range(80)]
+    expect = [orc.dinic_maxflow(g) for g in graphs]
+    srv = mf.Server(ctx, hbm_budget_bytes=32 << 20, max_inflight_launches=2)
+    try:
+        ids = {}
+        for i, (g, v) in enumerate(zip(graphs, expect)):
+            ids[srv.submit(g)] = v
+            if i % 10 == 9:
+                srv.progress()
         t0 = time.time()
         while ids and time.time() - t0 < 60:
             srv.progress(wait=True)

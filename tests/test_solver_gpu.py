@@ -8,7 +8,8 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import DEGENERATE, GOLDEN, ROOT, degenerate_graph, fixture_graph, generated_graph
+from conftest import (DEGENERATE, GOLDEN, ROOT, degenerate_graph, fixture_graph, generated_graph,
+                      random_small_graph)
 
 pytestmark = pytest.mark.gpu
 
@@ -79,20 +80,6 @@ def test_random_multigraphs(mf, orc, ctx, seed):
     for algo in ALGOS:
         for layout in LAYOUTS:
             solve_and_certify(mf, orc, ctx, g, algo, layout, expect)
-
-
-def random_small_graph(mf, rng):
-    n = int(rng.integers(2, 65))
-    m = int(rng.integers(0, 6 * n + 1))
-    u = rng.integers(0, n, size=m)
-    v = rng.integers(0, n, size=m)
-    s, t = (int(x) for x in rng.choice(n, size=2, replace=False))
-    top = int(rng.choice([1, 10, 1000, 2**31 - 1]))
-    cap = rng.integers(0, top, size=m, endpoint=True).astype(np.int64)
-    out_s = np.nonzero(u == s)[0]
-    if cap[out_s].sum() > 2**31 - 1:  # every excess must fit int32: scale the source arcs
-        cap[out_s] = cap[out_s] // (len(out_s) + 1)
-    return mf.Graph.from_arcs(n, u, v, cap.astype(np.int32), s, t)
 
 
 def test_random_small_graphs_random_options(mf, orc, ctx):
