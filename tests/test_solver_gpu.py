@@ -120,6 +120,34 @@ def test_random_small_graphs_batched(mf, orc, ctx):
         assert vals.tolist() == expect, (case, opts)
 
 
+def test_random_rmat_random_options(mf, orc, ctx):
+    # 16 power-law graphs (R-MAT scale 10-15, random skew, edge factor and seed: hubs of
+    # thousands of slots, grid-wide BFS levels), each on a random algorithm, layout and set of
+    # launch options: value equal to Dinic's, flow certified
+    rng = np.random.default_rng(2718)
+    for case in range(16):
+        a = float(rng.uniform(0.45, 0.65))
+        bc = (1.0 - a) * float(rng.uniform(0.7, 0.95)) / 2
+        g = mf.gen_rmat(int(rng.integers(10, 16)), int(rng.choice([4, 8, 16])), a, bc, bc,
+                        int(rng.choice([1, 100, 100000])), int(rng.integers(1, 1 << 30)))
+        algo = str(rng.choice(ALGOS))
+        layout = str(rng.choice(LAYOUTS))
+        opts = dict(cycles_per_relabel=int(rng.choice([0, 3, 12, 40])),
+                    bfs_block_frontier=int(rng.choice([0, 64, 1024])),
+                    bfs_direction=int(rng.choice([0, 1, 2])),
+                    forward_hops=int(rng.choice([-1, 0, 2])),
+                    bfs_cooperative=int(rng.choice([0, 1])),
+                    cycle_graphs=int(rng.choice([0, 1])))
+        if algo == "vc" and rng.random() < 0.5:
+            opts["class_lanes"] = [int(x) for x in rng.choice([4, 8, 16, 32], size=3)] + \
+                                  [int(rng.choice([32, 64])), int(rng.choice([64, 256]))]
+        expect = orc.dinic_maxflow(g)
+        try:
+            solve_and_certify(mf, orc, ctx, g, algo, layout, expect, **opts)
+        except AssertionError as err:
+            raise AssertionError(f"case {case}: n={g.n} m={g.m} {algo}/{layout} {opts}: {err}")
+
+
 def test_launch_geometry_stress(mf, orc, ctx, golden):
     # repeated solves with varied cadence, tile width and BFS hand-over: the lock-free
     # schedule differs every time, the value may not
