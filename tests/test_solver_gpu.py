@@ -148,6 +148,34 @@ def test_random_rmat_random_options(mf, orc, ctx):
             raise AssertionError(f"case {case}: n={g.n} m={g.m} {algo}/{layout} {opts}: {err}")
 
 
+def test_random_long_paths_random_options(mf, orc, ctx):
+    # 16 long-path graphs (Genrmf and Washington random level graphs of random shape, capacity
+    # range and seed: thousands of BFS levels in the one-workgroup and cooperative kernels), each
+    # on a random algorithm, layout and set of launch options, certified against Dinic
+    rng = np.random.default_rng(1618)
+    for case in range(16):
+        if case % 2 == 0:
+            g = mf.gen_genrmf(int(rng.integers(3, 20)), int(rng.integers(2, 24)),
+                              int(rng.integers(1, 10)), int(rng.choice([10, 1000, 100000])),
+                              int(rng.integers(1, 1 << 30)))
+        else:
+            g = mf.gen_washington_rlg(int(rng.integers(4, 64)), int(rng.integers(4, 160)),
+                                      int(rng.choice([10, 10000])), int(rng.integers(1, 1 << 30)))
+        algo = str(rng.choice(ALGOS))
+        layout = str(rng.choice(LAYOUTS))
+        opts = dict(cycles_per_relabel=int(rng.choice([0, 5, 100, 700])),
+                    bfs_block_frontier=int(rng.choice([0, 16, 256, 4096])),
+                    bfs_direction=int(rng.choice([0, 1])),
+                    forward_hops=int(rng.choice([-1, 2, 8])),
+                    bfs_cooperative=int(rng.choice([0, 1])),
+                    cycle_graphs=int(rng.choice([0, 1])))
+        expect = orc.dinic_maxflow(g)
+        try:
+            solve_and_certify(mf, orc, ctx, g, algo, layout, expect, **opts)
+        except AssertionError as err:
+            raise AssertionError(f"case {case}: n={g.n} m={g.m} {algo}/{layout} {opts}: {err}")
+
+
 def test_launch_geometry_stress(mf, orc, ctx, golden):
     # repeated solves with varied cadence, tile width and BFS hand-over: the lock-free
     # schedule differs every time, the value may not
