@@ -430,6 +430,110 @@ __device__ __forceinline__ void discharge_multi(const L &g, const CycleArgs &a, 
   queue_next(a, S, u, want_u);
 }
 
+// One discharge step of a hub by the whole block, without block barriers while it scans: each
+// wave takes every fourth chunk of kMultiBatch x 64 slots, loads the chunk's slot records and
+// neighbour heights, and reserves the residual of its arcs below h(u) on a block-shared counter
+// (one LDS atomic per wave and chunk).  Its share is what the reservations before it left of
+// e(u); it pushes that share along its arcs in slot order.  A wave stops once the counter shows
+// e(u) placed.  Waves therefore advance at their own pace instead of the slowest wave's, chunk
+// by chunk.  As in discharge_multi, only a full scan without any arc below h(u) relabels: the
+// block then takes the lowest (height, key) of all lanes, and every lane whose lowest arc
+// sits at the new height pushes along it.
+template <class L>
+__device__ __forceinline__ void discharge_block(const L &g, const CycleArgs &a, Stage *S, int u,
+                                                Stats &st) {
+  constexpr int B = kMultiBatch, W = kBlock / 64;
+  __shared__ unsigned long long s_placed;
+  __shared__ int s_eu;
+  const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __syncthreads();  // the block's previous hub is done with the shared words
+  if (tid == 0) {
+    s_eu = a.e[u];
+    s_placed = 0;
+  }
+  const int hu = a.h[u];
+  __syncthreads();
+  const int eu = s_eu;
+  if (hu >= a.H) return;  // became inactive after it was queued (uniform)
+  Cand keep;
+  int scanned = 0;
+#pragma unroll 1
+  for (int seg = 0; seg < L::kSeg; ++seg) {
+    const int2 r = g.range(u, seg);
+    for (int cs = r.x + wave * 64 * B; cs < r.y; cs += W * 64 * B) {
+      if (__hip_atomic_load(&s_placed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >=
+          (unsigned long long)eu)
+        break;  // wave-uniform: one LDS word
+      int w[B], res[B], key[B], hw[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        w[b] = u;
+        res[b] = 0;
+        key[b] = 0;
+        if (cs + lane + b * 64 < r.y) g.scan(seg, cs + lane + b * 64, w[b], res[b], key[b]);
+      }
+#pragma unroll
+      for (int b = 0; b < B; ++b) hw[b] = (res[b] > 0 && w[b] != u) ? a.h[w[b]] : 0;
+      long long want = 0;
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        if (res[b] <= 0 || w[b] == u) continue;
+        keep.consider(hw[b], key[b], w[b], res[b]);
+        if (hw[b] < hu) want += res[b];
+      }
+      scanned += min(r.y - cs, 64 * B);
+      const long long sum = tile_sum<64>(want);
+      long long d = 0;
+      if (sum > 0) {
+        unsigned long long got = 0;
+        if (lane == 0) got = atomicAdd(&s_placed, (unsigned long long)sum);
+        got = (unsigned long long)tile_first<64>((int)min(got, (unsigned long long)eu));
+        const long long share = max(0LL, min(sum, (long long)eu - (long long)got));
+        const long long incl = tile_scan<64>(want);
+        d = max(0LL, min(want, share - (incl - want)));
+      }
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        bool add = false;
+        if (d > 0 && res[b] > 0 && w[b] != u && hw[b] < hu) {
+          const int x = (int)min(d, (long long)res[b]);
+          d -= x;
+          add = push_to(g, a, w[b], key[b], hw[b], x, st);
+        }
+        queue_next(a, S, w[b], add);
+      }
+    }
+  }
+  if (lane == 0) st.slots += scanned;
+  __syncthreads();  // every wave's reservations are in
+  long long placed = (long long)min(s_placed, (unsigned long long)eu);
+  Plan p;
+  p.dead = false;
+  p.hnew = hu;
+  if (placed == 0 && eu > 0) {  // a full scan found no arc below h(u): relabel
+    plan_heights(block_min(keep.best), hu, a.H, p);
+    if (!p.dead) {
+      const long long want =
+          (keep.best != ~0ull && (int)(keep.best >> 32) < p.hnew) ? keep.res : 0;
+      long long sum;
+      const long long incl = block_scan(want, sum);
+      const int d = (int)max(0LL, min(want, (long long)eu - (incl - want)));
+      queue_next(a, S, keep.w, push_arc(g, a, keep, d, st));
+      placed = min((long long)eu, sum);
+    }
+  }
+  bool want_u = false;
+  if (tid == 0) {
+    st.active++;
+    if (p.dead || p.hnew > hu) {
+      a.h[u] = p.hnew;
+      st.relabels++;
+    }
+    if (!p.dead) want_u = atomicSub(&a.e[u], (int)placed) - placed > 0;
+  }
+  queue_next(a, S, u, want_u);
+}
+
 // One discharge step of u by T lanes; returns the vertex to continue with (forwarding) or -1.
 // The tile that takes a queued vertex stamps it with the cycle before its step, so no forwarder
 // discharges it a second time in the same cycle (finish_vertex).  The stamp's old value is
@@ -488,34 +592,41 @@ __device__ __forceinline__ void order_after(unsigned long long &x, unsigned long
 template <class L>
 __device__ void discharge_part(const L &g, const CycleArgs &a, Stage *S, int u, int p, int P,
                                int i, Stats &st) {
-  constexpr int B = kMultiBatch;
-  __shared__ long long s_share;
+  constexpr int B = kMultiBatch, W = kBlock / 64;
   __shared__ unsigned long long s_word;
+  __shared__ unsigned long long s_pushed;  // the part's pushes, summed over its waves
+  __shared__ int s_stop;                   // a wave's share fell short: e(u) is placed
   unsigned long long *gs = a.giant + (size_t)i * kGiantWords;
   const int hu = a.h[u];
   if (hu >= a.H) return;  // uniform over the parts: inactive since it was queued
-  const int lane = (int)threadIdx.x;
+  const int lane = (int)threadIdx.x, lane64 = lane & 63, wave = lane >> 6;
+  if (lane == 0) {
+    s_pushed = 0;
+    s_stop = 0;
+  }
+  // (the barriers of Group::first order the two stores above before any wave reads them)
   const int ep = Group<kBlock>::first(lane == 0 ? ld_acq(&a.e[u]) : 0);
   const int deg = slot_degree(g, u);
   const int lo = p * kPartSlots, hi = min(deg, lo + kPartSlots);
   Cand keep;               // this lane's lowest arc over the part's scanned slots
-  long long pushed = 0;    // the part's pushes (the same in every lane)
-  bool more = true;
   int scanned = 0, acc = 0;
+  // each wave scans every W-th chunk of 64 x B slots on its own (no block barrier per chunk):
+  // it reserves its chunk's admissible residual on word [2] and pushes its share
 #pragma unroll 1
   for (int seg = 0; seg < L::kSeg; ++seg) {
     const int2 r = g.range(u, seg);
     const int len = r.y - r.x;
     const int s0 = r.x + max(lo - acc, 0), s1 = r.x + min(hi - acc, len);
     acc += len;
-    for (int cs = s0; cs < s1 && more; cs += kBlock * B) {
+    for (int cs = s0 + wave * 64 * B; cs < s1; cs += W * 64 * B) {
+      if (__hip_atomic_load(&s_stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
       int w[B], res[B], key[B], hw[B];
 #pragma unroll
       for (int b = 0; b < B; ++b) {
         w[b] = u;
         res[b] = 0;
         key[b] = 0;
-        if (cs + lane + b * kBlock < s1) g.scan(seg, cs + lane + b * kBlock, w[b], res[b], key[b]);
+        if (cs + lane64 + b * 64 < s1) g.scan(seg, cs + lane64 + b * 64, w[b], res[b], key[b]);
       }
 #pragma unroll
       for (int b = 0; b < B; ++b) hw[b] = (res[b] > 0 && w[b] != u) ? a.h[w[b]] : 0;
@@ -526,21 +637,21 @@ __device__ void discharge_part(const L &g, const CycleArgs &a, Stage *S, int u, 
         keep.consider(hw[b], key[b], w[b], res[b]);
         if (hw[b] < hu) want += res[b];
       }
-      scanned += min(s1 - cs, kBlock * B);
-      long long sum;
-      const long long incl = block_scan(want, sum);
-      if (lane == 0) {
-        long long share = 0;
-        if (sum > 0) {
-          const long long got = (long long)atomicAdd(&gs[2], (unsigned long long)sum);
-          share = max(0LL, min(sum, (long long)ep - got));
+      scanned += min(s1 - cs, 64 * B);
+      const long long sum = tile_sum<64>(want);
+      long long d = 0;
+      if (sum > 0) {
+        unsigned long long got = 0;
+        if (lane64 == 0) got = atomicAdd(&gs[2], (unsigned long long)sum);
+        got = (unsigned long long)tile_first<64>((int)min(got, (unsigned long long)ep));
+        const long long share = max(0LL, min(sum, (long long)ep - (long long)got));
+        const long long incl = tile_scan<64>(want);
+        d = max(0LL, min(want, share - (incl - want)));
+        if (lane64 == 0) {
+          atomicAdd(&s_pushed, (unsigned long long)share);
+          if (share < sum) s_stop = 1;
         }
-        s_share = share;
       }
-      __syncthreads();
-      const long long share = s_share;
-      __syncthreads();
-      long long d = max(0LL, min(want, share - (incl - want)));
 #pragma unroll
       for (int b = 0; b < B; ++b) {
         bool add = false;
@@ -551,13 +662,12 @@ __device__ void discharge_part(const L &g, const CycleArgs &a, Stage *S, int u, 
         }
         queue_next(a, S, w[b], add);
       }
-      pushed += share;
-      more = share == sum;
     }
   }
-  const unsigned long long best = block_min(keep.best);
+  if (lane64 == 0) st.slots += scanned;
+  const unsigned long long best = block_min(keep.best);  // (its barriers: every wave is done)
+  const long long pushed = (long long)s_pushed;
   if (lane == 0) {
-    st.slots += scanned;
     const unsigned long long prev = atomicMin(&gs[0], best);
     unsigned long long report = (1ull << 32) | (unsigned long long)pushed;
     order_after(report, prev);
@@ -661,7 +771,7 @@ __device__ __forceinline__ void cycle_class(const L &g, const CycleArgs &a, Stag
         v = discharge<T>(g, a, S, v, lane, hop < a.fwd_hops ? c : -1, hop == 0 && a.fwd_hops > 0, st);
     }
   } else {
-    for (int i = b; i < cnt; i += nblk) discharge_multi<kBlock>(g, a, S, a.in.items[base + i], (int)threadIdx.x, st);
+    for (int i = b; i < cnt; i += nblk) discharge_block(g, a, S, a.in.items[base + i], st);
   }
 }
 
