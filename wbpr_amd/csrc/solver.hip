@@ -1814,7 +1814,7 @@ __device__ __forceinline__ void batch_cycle_class(const L &g, const CycleArgs &c
     MF_BATCH_CASE(64)
 #undef MF_BATCH_CASE
     default:
-      for (int i = 0; i < cnt; ++i) discharge_multi<kBlock>(g, ca, nullptr, ca.in.items[base + i], tid, st);
+      for (int i = 0; i < cnt; ++i) discharge_block(g, ca, nullptr, ca.in.items[base + i], st);
       break;
   }
 }
