@@ -339,7 +339,7 @@ __device__ __forceinline__ bool push_to(const L &g, const CycleArgs &a, int w, i
 }
 
 // One discharge step of a vertex by a group of T lanes that pushes along many arcs (a 64-lane
-// tile or the whole block), chunk by chunk.  A chunk is one batch of B slots per lane: the lane
+// tile; whole-block hubs use discharge_block, below), chunk by chunk.  A chunk is one batch of B slots per lane: the lane
 // loads its B slot records, then their B neighbour heights, and pushes its share of what is
 // left of e(u) (a prefix split over the lanes) along every one of its arcs that is lower than
 // h(u).  The scan stops as soon as the excess is placed, so the admissible arcs of the first
